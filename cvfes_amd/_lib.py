@@ -1,0 +1,105 @@
+"""ctypes binding of ``libcvfes_mi355x.so`` (C ABI declared in ``include/cvfes_mi355x.h``).
+
+``torch`` is imported before the library is opened so that the library's ``libamdhip64.so.7``
+dependency resolves to the HIP runtime torch already loaded (same SONAME): device pointers and
+streams handed over from torch tensors are then valid inside the library.  There is no fallback:
+if the library is missing, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (load torch's HIP runtime first)
+
+LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
+ABI_VERSION = 1
+
+METHOD_STORED = 0
+METHOD_MATFREE = 1
+METHODS = {"stored": METHOD_STORED, "bcsr": METHOD_STORED, "matfree": METHOD_MATFREE,
+           "ebe": METHOD_MATFREE}
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_int = ctypes.c_int
+
+
+class CvfesError(RuntimeError):
+    """A negative return code from the native library."""
+
+
+class CvfesSystem(ctypes.Structure):
+    """Mirror of ``cvfes_system_t``."""
+
+    _fields_ = [
+        ("n_nodes", _i64), ("n_smp", _i64), ("nnzb", _i64), ("n_elms", _i64),
+        ("indptr", _vp), ("indices", _vp), ("values", _vp),
+        ("cptr", _vp), ("celem", _vp), ("khat", _vp), ("coef", _vp),
+        ("lm", _vp), ("lm_sstride", _i64), ("f1", _vp), ("fixed", _vp), ("bdy_u", _f64),
+    ]
+
+
+class CvfesLoad(ctypes.Structure):
+    """Mirror of ``cvfes_load_t``."""
+
+    _fields_ = [("p", _vp), ("ga", _vp), ("qa", _vp), ("gb", _vp), ("qb", _vp)]
+
+
+_SIGNATURES = {
+    "cvfes_abi_version": (_int, []),
+    "cvfes_last_error": (ctypes.c_char_p, []),
+    "cvfes_stream_sync": (_int, [_vp]),
+    "cvfes_element_matrices": (_int, [_vp, _i64, _vp, _vp, _f64, _f64, _vp, _vp, _vp, _vp]),
+    "cvfes_element_coef": (_int, [_vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "cvfes_lumped_mass_load": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                      _f64, _f64, _vp, _vp]),
+    "cvfes_assemble_bcsr": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "cvfes_bcsr_spmv": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp]),
+    "cvfes_initial_state": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp,
+                                   ctypes.POINTER(CvfesLoad), _f64, _f64, _f64, _vp, _vp, _vp]),
+    "cvfes_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, ctypes.POINTER(_vp),
+                          ctypes.POINTER(CvfesLoad), _i64, _f64, _f64, _f64]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_LIB = None
+
+
+def lib(path: Path | str | None = None):
+    """Open (once) and return the native library with argument types declared."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise CvfesError(f"{p} is not built: run `python -m cvfes_amd.build` (hipcc, gfx950)")
+    handle = ctypes.CDLL(str(p))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    if handle.cvfes_abi_version() != ABI_VERSION:
+        raise CvfesError("libcvfes_mi355x.so ABI version mismatch; rebuild the library")
+    if path is None:
+        _LIB = handle
+    return handle
+
+
+def check(rc: int) -> None:
+    """Raise :class:`CvfesError` with the library's message when ``rc`` is an error code."""
+    if rc != 0:
+        msg = lib().cvfes_last_error().decode(errors="replace")
+        raise CvfesError(f"cvfes error {rc}: {msg}")
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a tensor, ``None`` for ``None``."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    """Raw ``hipStream_t`` of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,20 @@
+// Library-wide entry points: ABI version, error text, stream synchronisation.
+#include "common.h"
+
+namespace cvfes {
+std::string &last_error() {
+    static thread_local std::string msg;
+    return msg;
+}
+}  // namespace cvfes
+
+extern "C" int cvfes_abi_version(void) { return CVFES_ABI_VERSION; }
+
+extern "C" const char *cvfes_last_error(void) { return cvfes::last_error().c_str(); }
+
+extern "C" int cvfes_stream_sync(void *stream) {
+    hipError_t err = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (err != hipSuccess)
+        return cvfes::fail(CVFES_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(err));
+    return CVFES_OK;
+}

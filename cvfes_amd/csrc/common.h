@@ -1,0 +1,90 @@
+// Shared helpers of the gfx950 kernels: error reporting, launch geometry, vector loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "cvfes_mi355x.h"
+
+namespace cvfes {
+
+// Thread-local text of the last failure, returned by cvfes_last_error().
+std::string &last_error();
+
+inline int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    last_error() = buf;
+    return code;
+}
+
+inline int check_launch(const char *what) {
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+        return fail(CVFES_ERR_HIP, "%s: %s", what, hipGetErrorString(err));
+    return CVFES_OK;
+}
+
+#define CVFES_REQUIRE(cond, ...)                                  \
+    do {                                                          \
+        if (!(cond)) return ::cvfes::fail(CVFES_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+constexpr int kBlock = 256;    // 4 wavefronts of 64 lanes
+constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, each with its own L2
+
+// Grid size for `work` threads, padded to a multiple of the XCD count so that the
+// XCD-contiguous remapping below is a bijection.
+inline unsigned grid_for(int64_t work) {
+    int64_t blocks = (work + kBlock - 1) / kBlock;
+    blocks = (blocks + kNumXcd - 1) / kNumXcd * kNumXcd;
+    if (blocks < kNumXcd) blocks = kNumXcd;
+    return static_cast<unsigned>(blocks);
+}
+
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b % 8).  Remap so
+// that XCD k processes the k-th contiguous slice of the logical block range: consecutive block
+// rows then share one L2, which is where their common neighbour columns are cached.
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t b = blockIdx.x;
+    const int64_t per = gridDim.x / kNumXcd;
+    return (b % kNumXcd) * per + b / kNumXcd;
+}
+
+template <int VW>
+struct Vec {
+    double v[VW];
+};
+
+template <int VW>
+__device__ __forceinline__ Vec<VW> load_vec(const double *__restrict__ p) {
+    Vec<VW> r;
+    if constexpr (VW == 2) {
+        const double2 t = *reinterpret_cast<const double2 *>(p);
+        r.v[0] = t.x;
+        r.v[1] = t.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) r.v[k] = p[k];
+    }
+    return r;
+}
+
+template <int VW>
+__device__ __forceinline__ void store_vec(double *__restrict__ p, const Vec<VW> &r) {
+    if constexpr (VW == 2) {
+        *reinterpret_cast<double2 *>(p) = make_double2(r.v[0], r.v[1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) p[k] = r.v[k];
+    }
+}
+
+}  // namespace cvfes

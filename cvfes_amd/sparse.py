@@ -1,0 +1,90 @@
+"""Block-CSR structure of the membrane stiffness and the index maps the GPU kernels use.
+
+``SparseInfo`` restates CVFES's node-adjacency structure (``sparse.py:36-64``): row ``a`` of the
+3x3-block matrix holds the sorted, de-duplicated set of nodes that share an element with node
+``a`` (node ``a`` included).  ``indptr``/``indices`` are int64 as in the reference; the structure
+is built with vectorised sorting instead of per-node Python lists, with an identical result.
+
+``BlockMaps`` adds the integer maps that the MI355X kernels need on top of that structure:
+
+* ``cptr``/``celem``/``cab``: for every block ``p`` the list of element contributions
+  ``(e, a_local*3 + b_local)`` in increasing element order - the order in which the reference's
+  ``assembling`` (``optimizedSolidAssemble.pyx:133-188``) adds them into ``K[p]``;
+* ``cpos[e*9 + ab]``: the inverse map (where element ``e``'s block ``ab`` lands);
+* ``nptr``/``nent``: node -> incident (element, local index) lists, used to pull lumped mass and
+  nodal loads without scatter races.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = ["SparseInfo", "BlockMaps", "block_structure"]
+
+
+def block_structure(elements: np.ndarray, n_nodes: int):
+    """Return ``(indptr, indices)`` int64 of the node-adjacency block structure."""
+    elements = np.asarray(elements, dtype=np.int64)
+    rows = np.repeat(elements, 3, axis=1).ravel()
+    cols = np.tile(elements, (1, 3)).ravel()
+    keys = np.unique(rows * n_nodes + cols)
+    row_of = keys // n_nodes
+    indices = (keys % n_nodes).astype(np.int64)
+    indptr = np.zeros(n_nodes + 1, dtype=np.int64)
+    np.cumsum(np.bincount(row_of, minlength=n_nodes), out=indptr[1:])
+    return indptr, indices
+
+
+class SparseInfo:
+    """Block-CSR structure with the attributes of ``sparse.py:36-64``.
+
+    ``indptr [nNodes+1]``, ``indices [length]`` (int64), ``length`` (number of 3x3 blocks),
+    ``nNodes``, ``dof``, ``ndof``, ``dofArray``.
+    """
+
+    def __init__(self, mesh, dof=3):
+        elements = np.asarray(mesh.elementNodeIds, dtype=np.int64)
+        self.indptr, self.indices = block_structure(elements, mesh.nNodes)
+        self.length = int(self.indices.size)
+        self.nNodes = int(mesh.nNodes)
+        self.dof = int(dof)
+        self.ndof = self.nNodes * self.dof
+        self.dofArray = np.arange(dof)
+
+    def New(self):
+        return np.zeros((self.length, self.dof, self.dof))
+
+
+class BlockMaps:
+    """Integer maps between elements, nodes and blocks (all int32, ready for the GPU)."""
+
+    def __init__(self, elements: np.ndarray, n_nodes: int, indptr: np.ndarray, indices: np.ndarray):
+        elements = np.asarray(elements, dtype=np.int64)
+        n_elm = elements.shape[0]
+        nnzb = indices.size
+        if n_nodes * 3 >= 2 ** 31 or 9 * n_elm >= 2 ** 31 or nnzb >= 2 ** 31:
+            raise ValueError("mesh too large for 32-bit block maps")
+        rows = np.repeat(elements, 3, axis=1).ravel()          # a of (e, a, b), ab-major per element
+        cols = np.tile(elements, (1, 3)).ravel()
+        # Block index of each (row, col) pair: offset of col inside the row's sorted column list.
+        keys = rows * n_nodes + cols
+        row_keys = np.repeat(np.arange(n_nodes, dtype=np.int64), np.diff(indptr)) * n_nodes + indices
+        pos = np.searchsorted(row_keys, keys)
+        if np.any(pos >= nnzb) or np.any(row_keys[np.minimum(pos, nnzb - 1)] != keys):
+            raise ValueError("element pair missing from the block structure")
+        order = np.argsort(pos, kind="stable")
+        self.cptr = np.zeros(nnzb + 1, dtype=np.int64)
+        np.cumsum(np.bincount(pos, minlength=nnzb), out=self.cptr[1:])
+        self.celem = (order // 9).astype(np.int32)
+        self.cab = (order % 9).astype(np.int32)
+        self.cpos = np.empty(9 * n_elm, dtype=np.int32)
+        self.cpos[order] = np.arange(9 * n_elm, dtype=np.int32)
+        self.block_of = pos.astype(np.int32)                   # [E*9] block of (e, ab)
+        # Node -> (element, local) incidence.
+        flat = elements.ravel()
+        norder = np.argsort(flat, kind="stable")
+        self.nptr = np.zeros(n_nodes + 1, dtype=np.int64)
+        np.cumsum(np.bincount(flat, minlength=n_nodes), out=self.nptr[1:])
+        self.nent = norder.astype(np.int32)                    # e*3 + local
+        self.n_contrib = 9 * n_elm
+        self.cptr = self.cptr.astype(np.int32)
+        self.nptr = self.nptr.astype(np.int32)

@@ -1,0 +1,286 @@
+"""Device-resident membrane system: structure, element blocks, mass, load and stiffness values.
+
+``MembraneSystem`` owns every device buffer of one mesh and its sample ensemble as torch tensors
+and drives the native kernels through the C ABI.  The host side does integer preprocessing only
+(block structure, element/block/node index maps); all floating-point work - element matrices,
+stiffness coefficients, lumped mass, unit load, block assembly, products and time steps - runs in
+the HIP kernels of ``libcvfes_mi355x.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import CvfesLoad, CvfesSystem, check, ptr
+from .sparse import BlockMaps, block_structure
+
+__all__ = ["MembraneSystem", "LoadTables", "MASS_SCALE"]
+
+# Lumped-mass normalisation: the CPU path's double quadrature weight gives rho*A/9*(...)
+# (optimizedSolidAssemble.pyx:313-323 with physicsSolver.py:197); the GPU kernels use rho*A/3*(...)
+# (physicsSolverSDampGPUs.cl:172-221).
+MASS_SCALE = {"cpu": 1.0 / 9.0, "gpu": 1.0 / 3.0}
+
+
+def _dev(a, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device)
+
+
+class LoadTables:
+    """Device load tables ``f_n = p[n] f1 + qa[n] ga + qb[n] gb`` for a run of steps."""
+
+    def __init__(self, p, device, ga=None, qa=None, gb=None, qb=None):
+        f64 = torch.float64
+        self.p = p.to(device, f64).contiguous() if isinstance(p, torch.Tensor) else _dev(np.atleast_1d(np.asarray(p, np.float64)), f64, device)
+        self.n = int(self.p.numel())
+        self.ga = None if ga is None else _dev(np.asarray(ga, np.float64).reshape(-1, 3), f64, device)
+        self.gb = None if gb is None else _dev(np.asarray(gb, np.float64).reshape(-1, 3), f64, device)
+        self.qa = None if qa is None else _dev(np.asarray(qa, np.float64).reshape(-1), f64, device)
+        self.qb = None if qb is None else _dev(np.asarray(qb, np.float64).reshape(-1), f64, device)
+        for q in (self.qa, self.qb):
+            if q is not None and q.numel() != self.n:
+                raise ValueError("load tables must all have one entry per step")
+        if (self.ga is None) != (self.qa is None) or (self.gb is None) != (self.qb is None):
+            raise ValueError("a traction field needs its table and vice versa")
+
+    def struct(self, offset=0):
+        """``cvfes_load_t`` whose step 0 is entry ``offset`` of the tables."""
+        if not 0 <= offset <= self.n:
+            raise ValueError("load table offset out of range")
+        o = 8 * offset
+        return CvfesLoad(p=self.p.data_ptr() + o,
+                         ga=ptr(self.ga), qa=None if self.qa is None else self.qa.data_ptr() + o,
+                         gb=ptr(self.gb), qb=None if self.qb is None else self.qb.data_ptr() + o)
+
+
+class MembraneSystem:
+    """Assembled ensemble membrane system on one GPU.
+
+    Parameters
+    ----------
+    nodes, elements, boundary : reference coordinates ``[N,3]``, triangles ``[E,3]``, fixed nodes.
+    n_smp : number of samples ``S``.
+    vthickness : ``[N,S]`` or ``[N]`` node thickness.
+    vE / elmE : node modulus ``[N,S]`` or element modulus ``[E,S]`` (one of them).
+    coef : optional precomputed stiffness coefficient ``[E,S]`` (overrides vE/elmE).
+    density, nu, kshear : material constants (``kshear`` = 5/6 as ``physicsSolver.py:177``).
+    mass_form : ``"cpu"`` (oracle parity, default) or ``"gpu"`` lumping.
+    store_values : form the stored block-CSR values ``[nnzb][9][S]``.
+    """
+
+    def __init__(self, nodes, elements, boundary, n_smp, *, vthickness, vE=None, elmE=None,
+                 coef=None, density=1.0, nu=0.5, kshear=5.0 / 6.0, mass_form="cpu", bdy_u=0.0,
+                 store_values=True, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("MembraneSystem needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.lib()
+        self.device = torch.device(device if device is not None else "cuda")
+        dev = self.device
+        nodes = np.asarray(nodes, np.float64)
+        elements = np.asarray(elements, np.int64)
+        self.n_nodes = int(nodes.shape[0])
+        self.n_elms = int(elements.shape[0])
+        self.n_smp = int(n_smp)
+        self.nu, self.kshear, self.density = float(nu), float(kshear), float(density)
+        # --- integer preprocessing (host) --------------------------------------------------------
+        self.indptr_h, self.indices_h = block_structure(elements, self.n_nodes)
+        self.nnzb = int(self.indices_h.size)
+        maps = BlockMaps(elements, self.n_nodes, self.indptr_h, self.indices_h)
+        self.maps = maps
+        i32 = torch.int32
+        self.nodes = _dev(nodes, torch.float64, dev)
+        self.elems = _dev(elements, i32, dev)
+        self.indptr = _dev(self.indptr_h, i32, dev)
+        self.indices = _dev(self.indices_h, i32, dev)
+        self.cptr = _dev(maps.cptr, i32, dev)
+        self.celem = _dev(maps.celem, i32, dev)
+        self.cpos = _dev(maps.cpos, i32, dev)
+        self.nptr = _dev(maps.nptr, i32, dev)
+        self.nent = _dev(maps.nent, i32, dev)
+        fixed = np.zeros(self.n_nodes, np.uint8)
+        fixed[np.asarray(boundary, np.int64)] = 1
+        self.fixed = _dev(fixed, torch.uint8, dev)
+        self.boundary = np.asarray(boundary, np.int64)
+        self.bdy_u = float(bdy_u)
+        f64 = torch.float64
+        S = self.n_smp
+        with torch.cuda.device(dev):
+            st = _lib.stream_handle(dev)
+            # --- element blocks ------------------------------------------------------------------
+            self.khat = torch.empty((9 * self.n_elms, 9), dtype=f64, device=dev)
+            self.area = torch.empty(self.n_elms, dtype=f64, device=dev)
+            self.normal = torch.empty((self.n_elms, 3), dtype=f64, device=dev)
+            check(self.lib.cvfes_element_matrices(st, self.n_elms, ptr(self.nodes), ptr(self.elems),
+                                                  self.nu, self.kshear, ptr(self.cpos),
+                                                  ptr(self.khat), ptr(self.area), ptr(self.normal)))
+            # --- thickness and stiffness coefficient ---------------------------------------------
+            vthick = np.asarray(vthickness, np.float64)
+            if vthick.ndim == 2 and vthick.shape[1] == S and np.all(vthick == vthick[:, :1]):
+                vthick = vthick[:, 0]
+            self.thick_sstride = 1 if vthick.ndim == 2 else 0
+            self.vthick = _dev(vthick, f64, dev)
+            if coef is not None:
+                self.coef = _dev(np.asarray(coef, np.float64).reshape(self.n_elms, S), f64, dev)
+            else:
+                vE_d = None if vE is None else _dev(np.asarray(vE, np.float64).reshape(self.n_nodes, S), f64, dev)
+                elmE_d = None if elmE is None else _dev(np.asarray(elmE, np.float64).reshape(self.n_elms, S), f64, dev)
+                if vE_d is None and elmE_d is None:
+                    raise ValueError("one of coef, vE, elmE is required")
+                self.coef = torch.empty((self.n_elms, S), dtype=f64, device=dev)
+                check(self.lib.cvfes_element_coef(st, self.n_elms, S, ptr(self.elems), ptr(self.vthick),
+                                                  self.thick_sstride, ptr(vE_d), ptr(elmE_d),
+                                                  ptr(self.coef)))
+                torch.cuda.current_stream(dev).synchronize()
+            # --- lumped mass and unit-pressure load -----------------------------------------------
+            self.mass_form = mass_form
+            lm_shape = (self.n_nodes, S) if self.thick_sstride else (self.n_nodes,)
+            self.lm = torch.empty(lm_shape, dtype=f64, device=dev)
+            self.f1 = torch.empty((self.n_nodes, 3), dtype=f64, device=dev)
+            check(self.lib.cvfes_lumped_mass_load(st, self.n_nodes, S, ptr(self.nptr), ptr(self.nent),
+                                                  ptr(self.elems), ptr(self.area), ptr(self.normal),
+                                                  ptr(self.vthick), self.thick_sstride, self.density,
+                                                  MASS_SCALE[mass_form], ptr(self.lm), ptr(self.f1)))
+            # --- stored block-CSR values ---------------------------------------------------------
+            self.values = None
+            if store_values:
+                self.assemble_values()
+        self._sys = self._make_struct()
+        self.state = None
+
+    # ------------------------------------------------------------------------------------------
+    def assemble_values(self):
+        """Form (or re-form) the stored block values ``[nnzb][9][S]`` from ``coef``."""
+        S = self.n_smp
+        if self.values is None:
+            self.values = torch.empty((self.nnzb, 9, S), dtype=torch.float64, device=self.device)
+        check(self.lib.cvfes_assemble_bcsr(self.stream, self.nnzb, S, ptr(self.cptr), ptr(self.celem),
+                                           ptr(self.khat), ptr(self.coef), ptr(self.values)))
+        if hasattr(self, "_sys"):
+            self._sys = self._make_struct()
+
+    def release_values(self):
+        """Drop the stored values (matrix-free runs only need coef and khat)."""
+        self.values = None
+        self._sys = self._make_struct()
+
+    def _make_struct(self):
+        return CvfesSystem(
+            n_nodes=self.n_nodes, n_smp=self.n_smp, nnzb=self.nnzb, n_elms=self.n_elms,
+            indptr=ptr(self.indptr), indices=ptr(self.indices), values=ptr(self.values),
+            cptr=ptr(self.cptr), celem=ptr(self.celem), khat=ptr(self.khat), coef=ptr(self.coef),
+            lm=ptr(self.lm), lm_sstride=self.thick_sstride, f1=ptr(self.f1), fixed=ptr(self.fixed),
+            bdy_u=self.bdy_u)
+
+    @property
+    def stream(self):
+        return _lib.stream_handle(self.device)
+
+    @staticmethod
+    def method_id(method):
+        if isinstance(method, int):
+            return method
+        return _lib.METHODS[method]
+
+    def state_shape(self):
+        return (self.n_nodes, 3, self.n_smp)
+
+    # ------------------------------------------------------------------------------------------
+    def spmv(self, x, method="stored", out=None):
+        """``y = K x`` for ``x [N,3,S]`` (device tensor)."""
+        x = x.contiguous()
+        y = torch.empty_like(x) if out is None else out
+        check(self.lib.cvfes_bcsr_spmv(self.stream, ctypes.byref(self._sys), self.method_id(method),
+                                       ptr(x), ptr(y)))
+        return y
+
+    def loads(self, p, ga=None, qa=None, gb=None, qb=None):
+        """Upload load tables (see :class:`LoadTables`)."""
+        return LoadTables(p, self.device, ga=ga, qa=qa, gb=gb, qb=qb)
+
+    def initial_state(self, u0, du0, load, dt, beta=0.0, damp=0.0, method="stored", offset=0):
+        """Allocate the three state buffers and set ``u_0``, ``u_{-1}`` (``u0``/``du0`` are ``[3N]``).
+
+        ``load`` is a :class:`LoadTables` (or a pressure value/array); its entry ``offset`` is the
+        load at the start time.
+        """
+        f64 = torch.float64
+        if not isinstance(load, LoadTables):
+            load = self.loads(load)
+        shp = self.state_shape()
+        bufs = [torch.empty(shp, dtype=f64, device=self.device) for _ in range(3)]
+        u0_d = _dev(np.asarray(u0, np.float64).reshape(-1), f64, self.device)
+        du0_d = _dev(np.asarray(du0, np.float64).reshape(-1), f64, self.device)
+        ls = load.struct(offset)
+        check(self.lib.cvfes_initial_state(self.stream, ctypes.byref(self._sys), self.method_id(method),
+                                           ptr(u0_d), ptr(du0_d), ctypes.byref(ls), float(dt),
+                                           float(beta), float(damp), ptr(bufs[0]), ptr(bufs[1]),
+                                           ptr(bufs[2])))
+        torch.cuda.current_stream(self.device).synchronize()
+        self.state = bufs
+        return bufs
+
+    def step(self, load, dt, beta=0.0, damp=0.0, method="stored", offset=0, nsteps=None):
+        """Advance ``nsteps`` steps using load-table entries ``offset ... offset+nsteps-1``.
+
+        ``load`` is a :class:`LoadTables` or a host/device array of pressures.
+        """
+        if self.state is None:
+            raise RuntimeError("call initial_state() first")
+        if not isinstance(load, LoadTables):
+            load = self.loads(load)
+        n = load.n - offset if nsteps is None else int(nsteps)
+        if offset < 0 or n < 0 or offset + n > load.n:
+            raise ValueError("steps exceed the load tables")
+        ls = load.struct(offset)
+        arr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in self.state])
+        check(self.lib.cvfes_step(self.stream, ctypes.byref(self._sys), self.method_id(method), arr,
+                                  ctypes.byref(ls), n, float(dt), float(beta), float(damp)))
+        addr = {b.data_ptr(): b for b in self.state}
+        self.state = [addr[arr[k]] for k in range(3)]
+        self._load_keepalive = load
+        return self.state[0]
+
+    @property
+    def u(self):
+        return self.state[0]
+
+    @property
+    def up(self):
+        return self.state[1]
+
+    # ------------------------------------------------------------------------------------------
+    def values_reference_layout(self):
+        """Stored values as the reference's ``[nnzb, S, 3, 3]`` (host numpy)."""
+        v = self.values.cpu().numpy()
+        return np.ascontiguousarray(v.reshape(self.nnzb, 3, 3, self.n_smp).transpose(0, 3, 1, 2))
+
+    def lumped_mass_dofs(self):
+        """Lumped mass as the reference's ``[ndof, S]`` (host numpy)."""
+        lm = self.lm.cpu().numpy()
+        if lm.ndim == 1:
+            lm = np.repeat(lm[:, None], self.n_smp, axis=1)
+        return np.repeat(lm, 3, axis=0)
+
+    def critical_dt(self, beta=0.0, iters=60, seed=0, method="stored"):
+        """Largest stable step from a power iteration on ``M^-1 K`` (max over samples).
+
+        The central-difference recurrence with backward-difference stiffness damping is stable for
+        ``dt^2 lam + 2 dt beta lam < 4``, i.e. ``dt < -beta + sqrt(beta^2 + 4/lam)``.  The full
+        operator (fixed rows included) bounds the constrained one from above, so the bound is safe.
+        """
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        x = torch.rand(self.state_shape(), generator=g, dtype=torch.float64).to(self.device)
+        lm = self.lm
+        lm_b = lm[:, None, :] if lm.ndim == 2 else lm[:, None, None]
+        lam = None
+        for _ in range(iters):
+            kx = self.spmv(x, method)
+            # Rayleigh quotient of the pencil (K, M): never above lam_max, converges from below.
+            lam = (x * kx).sum(dim=(0, 1)) / (lm_b * x * x).sum(dim=(0, 1))
+            y = kx / lm_b
+            x = y / torch.linalg.vector_norm(y, dim=(0, 1))
+        lam_max = float(lam.max())
+        return float(-beta + np.sqrt(beta * beta + 4.0 / lam_max)), lam_max

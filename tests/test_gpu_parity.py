@@ -1,0 +1,134 @@
+"""GPU path vs the CPU oracle on the 64x32 tube ensemble (and variants) through the C ABI.
+
+Bars: block structure bit-exact; assembled values, lumped mass and load within 1e-13 relative;
+displacements after N steps within 1e-10 relative max-norm (max |u_gpu - u_cpu| / max |u_cpu|).
+"""
+import numpy as np
+import pytest
+
+from cvfes_amd.configs import get_case
+from oracle import solid_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+REL_U = 1e-10
+REL_ASSEMBLY = 1e-13
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+class Case:
+    """Inputs for one ensemble shared by the GPU system and the oracle."""
+
+    def __init__(self, name, n_smp=None, seed=None, node_E=False):
+        import torch  # noqa: F401
+        from cvfes_amd.system import MembraneSystem
+
+        c = get_case(name)
+        self.case = c
+        self.S = n_smp or c.n_smp
+        self.nodes, self.el, self.bdy = c.geometry()
+        self.N = self.nodes.shape[0]
+        rng = np.random.default_rng(seed if seed is not None else c.seed)
+        if node_E:
+            self.vE = c.E0 * np.exp(0.1 * rng.standard_normal((self.N, self.S)))
+            self.elmE = None
+        else:
+            from cvfes_amd.gmrf import sample_element_E
+            self.elmE = sample_element_E(self.el, self.S, seed=c.seed).E
+            self.vE = np.full((self.N, self.S), c.E0)
+        self.h = np.full((self.N, self.S), c.thickness)
+        self.sys = MembraneSystem(self.nodes, self.el, self.bdy, self.S, vthickness=self.h,
+                                  vE=self.vE if node_E else None, elmE=self.elmE,
+                                  density=c.density, nu=c.nu)
+        # Oracle assembly in the reference layout.
+        self.indptr, self.indices = so.sparse_info_loops(self.el, self.N)
+        elmGE, elmGThick = so.gauss_properties(self.vE, self.h, self.el)
+        coefK = so.coef_k(elmGThick, elmGE) if node_E else so.coef_k_element_E(elmGThick, self.elmE)
+        self.coefK = coefK
+        self.M, self.K, self.f1, self.LM = so.assemble_reference(
+            self.nodes, self.el, 1.0, so.d_matrix(c.nu), coefK, elmGThick, c.density,
+            self.indptr, self.indices)
+        self.bdy_dofs = (3 * self.bdy[:, None] + np.arange(3)).ravel()
+
+
+@pytest.fixture(scope="module")
+def case1(require_gpu):
+    return Case("tube-64x32-s16")
+
+
+def test_structure_bit_exact(case1):
+    assert np.array_equal(case1.sys.indptr.cpu().numpy().astype(np.int64), case1.indptr)
+    assert np.array_equal(case1.sys.indices.cpu().numpy().astype(np.int64), case1.indices)
+
+
+def test_coef_matches_oracle(case1):
+    assert rel_err(case1.sys.coef.cpu().numpy(), case1.coefK) < REL_ASSEMBLY
+
+
+def test_assembled_values_match_oracle(case1):
+    assert rel_err(case1.sys.values_reference_layout(), case1.K) < REL_ASSEMBLY
+
+
+def test_mass_and_load_match_oracle(case1):
+    assert rel_err(case1.sys.lumped_mass_dofs(), case1.LM) < REL_ASSEMBLY
+    assert rel_err(case1.sys.f1.cpu().numpy().ravel(), case1.f1) < REL_ASSEMBLY
+
+
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_spmv_matches_oracle(case1, method):
+    import torch
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((case1.N, 3, case1.S))
+    y = case1.sys.spmv(torch.as_tensor(x).cuda(), method).cpu().numpy()
+    yref = so.multiply_by_vector(case1.indptr, case1.indices, case1.K, x.reshape(-1, case1.S))
+    assert rel_err(y.reshape(-1, case1.S), yref) < 1e-13
+
+
+def _run_both(case, nsteps, method, beta=0.0, damp=0.0, dt=None, record=()):
+    c = case.case
+    dt = c.dt if dt is None else dt
+    p = c.pressures(0, nsteps, dt)
+    u0 = np.zeros(3 * case.N)
+    sysm = case.sys
+    sysm.initial_state(u0, u0, sysm.loads(p), dt, beta=beta, damp=damp, method=method)
+    out = {}
+    done = 0
+    for k in sorted(set(record) | {nsteps}):
+        sysm.step(p, dt, beta=beta, damp=damp, method=method, offset=done, nsteps=k - done)
+        done = k
+        out[k] = sysm.u.cpu().numpy().reshape(-1, case.S)
+    _, _, snaps = so.run(case.indptr, case.indices, case.K, case.LM, case.f1, p, u0, u0, dt,
+                         beta=beta, damp=damp, bdy_dofs=case.bdy_dofs, bdy_u=0.0,
+                         record=set(record) | {nsteps})
+    return out, snaps
+
+
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_1000_steps_match_oracle(case1, method):
+    out, snaps = _run_both(case1, 1000, method, record=(1, 10, 100))
+    for k, u in out.items():
+        assert np.all(np.isfinite(u))
+        assert rel_err(u, snaps[k]) < REL_U, f"step {k}"
+    # The motion is not trivial: the tube inflates.
+    assert np.max(np.abs(snaps[1000])) > 1e-6
+
+
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_damped_steps_match_oracle(case1, method):
+    out, snaps = _run_both(case1, 300, method, beta=1e-6, damp=50.0, dt=3e-6, record=(1, 50))
+    for k, u in out.items():
+        assert rel_err(u, snaps[k]) < REL_U, f"step {k}"
+
+
+def test_node_field_modulus_and_odd_sample_count(require_gpu):
+    c = Case("tube-64x32-s16", n_smp=5, node_E=True)
+    assert rel_err(c.sys.coef.cpu().numpy(), c.coefK) < REL_ASSEMBLY
+    out, snaps = _run_both(c, 200, "stored", record=(7,))
+    for k, u in out.items():
+        assert rel_err(u, snaps[k]) < REL_U
+    out, snaps = _run_both(c, 200, "matfree", record=(7,))
+    for k, u in out.items():
+        assert rel_err(u, snaps[k]) < REL_U
