@@ -20,6 +20,12 @@ METHOD_MATFREE = 1
 METHODS = {"stored": METHOD_STORED, "bcsr": METHOD_STORED, "matfree": METHOD_MATFREE,
            "ebe": METHOD_MATFREE}
 
+VARIANT_AUTO, VARIANT_GENERIC, VARIANT_ROWLOAD, VARIANT_ROWLOAD_NT = 0, 1, 2, 3
+VARIANT_ROWLOAD1, VARIANT_ROWLOAD1_NT = 4, 5
+VARIANTS = {"auto": VARIANT_AUTO, "generic": VARIANT_GENERIC, "rowload": VARIANT_ROWLOAD,
+            "rowload_nt": VARIANT_ROWLOAD_NT, "rowload1": VARIANT_ROWLOAD1,
+            "rowload1_nt": VARIANT_ROWLOAD1_NT}
+
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _f64 = ctypes.c_double
@@ -38,6 +44,8 @@ class CvfesSystem(ctypes.Structure):
         ("indptr", _vp), ("indices", _vp), ("values", _vp),
         ("cptr", _vp), ("celem", _vp), ("khat", _vp), ("coef", _vp),
         ("lm", _vp), ("lm_sstride", _i64), ("f1", _vp), ("fixed", _vp), ("bdy_u", _f64),
+        ("max_row_blocks", _i64), ("max_offdiag_contrib", _i64), ("max_diag_contrib", _i64),
+        ("variant", _i64),
     ]
 
 

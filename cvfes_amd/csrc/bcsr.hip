@@ -179,6 +179,181 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a, int64_t step) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Row step with every load of a block row issued up front (wave-uniform rows, two samples per lane).
+//
+// The generic k_step walks a row block by block, and for the matrix-free operator each element
+// contribution is a dependent chain (scalar load of the element id -> vector load of its
+// coefficient -> wait), so a wave has only a few loads in flight.  Here all addresses of a row are
+// formed first and every load is issued before any arithmetic, without branches between them:
+// block slots b < MAXB are clamped to the row's last block and masked, element slots are clamped
+// to the block's last contribution and masked.  Clamping keeps every access inside the row's own
+// blocks and contributions.  Limits (exact maxima over the mesh, computed on the host):
+//   blocks per row <= MAXB, contributions per off-diagonal block <= 2,
+//   contributions per diagonal block <= MAXB - 1 (a node has at least as many neighbours as
+//   incident triangles in a manifold triangulation).
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <int VW> struct VecT;
+template <> struct VecT<1> { typedef double type; };
+template <> struct VecT<2> { typedef dbl2 type; };
+
+template <int VW, bool NT>
+__device__ __forceinline__ typename VecT<VW>::type ldv(const double *p) {
+    typedef typename VecT<VW>::type V;
+    if constexpr (NT)
+        return __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+    else
+        return *reinterpret_cast<const V *>(p);
+}
+
+template <int VW>
+__device__ __forceinline__ typename VecT<VW>::type splat(double x) {
+    if constexpr (VW == 2) return dbl2{x, x};
+    else return x;
+}
+
+template <int VW>
+__device__ __forceinline__ typename VecT<VW>::type sel(bool keep, typename VecT<VW>::type v) {
+    return keep ? v : splat<VW>(0.0);
+}
+
+template <bool MATFREE, bool BETA, bool DAMP, bool NT, int MAXB, int VW>
+__global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t step) {
+    constexpr int MAXD = MAXB - 1;
+    typedef typename VecT<VW>::type V;
+    int32_t row;
+    int64_t s;
+    if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s)) return;
+    const RowArgs &k = a.k;
+    const int64_t S = k.n_smp;
+    const int32_t p0 = k.indptr[row];
+    const int32_t nb = k.indptr[row + 1] - p0;
+
+    // own state (also needed by the update)
+    V uo[3], po[3], wo[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int64_t off = (static_cast<int64_t>(row) * 3 + i) * S + s;
+        uo[i] = ldv<VW, false>(a.u + off);
+        po[i] = ldv<VW, false>(a.up + off);
+        wo[i] = BETA ? uo[i] + a.bdt * (uo[i] - po[i]) : uo[i];
+    }
+    V acc[3] = {splat<VW>(0.0), splat<VW>(0.0), splat<VW>(0.0)};
+    if (nb > 0) {
+        // neighbour states for every block slot
+        int32_t col[MAXB];
+        V w[MAXB][3];
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const int32_t pb = p0 + (b < nb ? b : nb - 1);
+            col[b] = k.indices[pb];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int64_t off = (static_cast<int64_t>(col[b]) * 3 + j) * S + s;
+                const V xu = ldv<VW, false>(a.u + off);
+                if constexpr (BETA) {
+                    const V xp = ldv<VW, false>(a.up + off);
+                    w[b][j] = xu + a.bdt * (xu - xp);
+                } else {
+                    w[b][j] = xu;
+                }
+            }
+        }
+        if constexpr (!MATFREE) {
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) {
+                const bool valid = b < nb;
+                const int32_t pb = p0 + (valid ? b : nb - 1);
+                const double *base = k.values + static_cast<int64_t>(pb) * 9 * S + s;
+                V kv[9];
+#pragma unroll
+                for (int ij = 0; ij < 9; ++ij) kv[ij] = ldv<VW, NT>(base + ij * S);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) acc[i] += sel<VW>(valid, kv[3 * i + j]) * w[b][j];
+            }
+        } else {
+            // diagonal block: the contributions of every element incident to the row node
+            int32_t pd = p0;
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) pd += (b < nb && col[b] < row) ? 1 : 0;
+            const int32_t cd0 = k.cptr[pd];
+            const int32_t nd = k.cptr[pd + 1] - cd0;
+            V cfd[MAXD];
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) {
+                const int32_t c = cd0 + (d < nd ? d : nd - 1);
+                cfd[d] = sel<VW>(d < nd, ldv<VW, false>(k.coef + static_cast<int64_t>(k.celem[c]) * S + s));
+            }
+            // off-diagonal blocks: one or two contributions each
+            V cfo[MAXB][2];
+            int32_t co[MAXB], no[MAXB];
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) {
+                const bool valid = b < nb && col[b] != row;
+                const int32_t pb = p0 + (b < nb ? b : nb - 1);
+                co[b] = k.cptr[pb];
+                no[b] = valid ? k.cptr[pb + 1] - co[b] : 0;
+                const int32_t c1 = co[b] + (no[b] > 1 ? 1 : 0);
+                cfo[b][0] = sel<VW>(no[b] > 0, ldv<VW, false>(k.coef + static_cast<int64_t>(k.celem[co[b]]) * S + s));
+                cfo[b][1] = sel<VW>(no[b] > 1, ldv<VW, false>(k.coef + static_cast<int64_t>(k.celem[c1]) * S + s));
+            }
+            {
+                V kv[9];
+#pragma unroll
+                for (int ij = 0; ij < 9; ++ij) kv[ij] = splat<VW>(0.0);
+#pragma unroll
+                for (int d = 0; d < MAXD; ++d) {
+                    const double *kh = k.khat + 9 * static_cast<int64_t>(cd0 + (d < nd ? d : nd - 1));
+#pragma unroll
+                    for (int ij = 0; ij < 9; ++ij) kv[ij] += cfd[d] * kh[ij];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) acc[i] += kv[3 * i + j] * wo[j];
+            }
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) {
+                const double *kh0 = k.khat + 9 * static_cast<int64_t>(co[b]);
+                const double *kh1 = k.khat + 9 * static_cast<int64_t>(co[b] + (no[b] > 1 ? 1 : 0));
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[i] += (cfo[b][0] * kh0[3 * i + j] + cfo[b][1] * kh1[3 * i + j]) * w[b][j];
+            }
+        }
+    }
+    const double pn = a.load.p[step];
+    const double qa = a.load.ga ? a.load.qa[step] : 0.0;
+    const double qb = a.load.gb ? a.load.qb[step] : 0.0;
+    const bool fixed = a.fixed != nullptr && a.fixed[row] != 0;
+    const V m = a.lm_sstride ? ldv<VW, false>(a.lm + static_cast<int64_t>(row) * S + s)
+                             : splat<VW>(a.lm[row]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int64_t d = 3 * static_cast<int64_t>(row) + i;
+        double fi = pn * a.f1[d];
+        if (a.load.ga) fi += qa * a.load.ga[d];
+        if (a.load.gb) fi += qb * a.load.gb[d];
+        V force = splat<VW>(fi);
+        if constexpr (DAMP) force -= a.ddt * (uo[i] - po[i]);
+        const V rhs = a.dt2 * (force - acc[i]) + m * (2.0 * uo[i] - po[i]);
+        V un;
+        if constexpr (VW == 2) {
+            un.x = m.x != 0.0 ? rhs.x / m.x : 0.0;
+            un.y = m.y != 0.0 ? rhs.y / m.y : 0.0;
+        } else {
+            un = m != 0.0 ? rhs / m : 0.0;
+        }
+        if (fixed) un = splat<VW>(a.bdy_u);
+        *reinterpret_cast<V *>(a.out + (static_cast<int64_t>(row) * 3 + i) * S + s) = un;
+    }
+}
+
 // values[p][ij][s] = sum_c coef[celem[c]][s] * khat[c][ij]; one thread per (block, sample).
 __global__ __launch_bounds__(kBlock) void k_assemble(int64_t nnzb, int64_t n_smp,
                                                       const int32_t *__restrict__ cptr,
@@ -311,7 +486,59 @@ void step_u(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool dam
         step_bd<MF, false, VW>(st, a, step, beta, damp);
 }
 
-void step_dispatch(hipStream_t st, bool mf, const StepArgs &a, int64_t step, bool beta, bool damp) {
+template <bool MF, bool NT, int MAXB, int VW>
+void launch_rowload(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
+    const int64_t work = a.k.n_nodes * (a.k.n_smp / VW);
+    const dim3 g(grid_for(work)), b(kBlock);
+    if (beta) {
+        if (damp) hipLaunchKernelGGL((k_step_rowload<MF, true, true, NT, MAXB, VW>), g, b, 0, st, a, step);
+        else hipLaunchKernelGGL((k_step_rowload<MF, true, false, NT, MAXB, VW>), g, b, 0, st, a, step);
+    } else {
+        if (damp) hipLaunchKernelGGL((k_step_rowload<MF, false, true, NT, MAXB, VW>), g, b, 0, st, a, step);
+        else hipLaunchKernelGGL((k_step_rowload<MF, false, false, NT, MAXB, VW>), g, b, 0, st, a, step);
+    }
+}
+
+template <bool MF, bool NT, int VW>
+void rowload_maxb(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp, int maxb) {
+    if (maxb == 7) launch_rowload<MF, NT, 7, VW>(st, a, step, beta, damp);
+    else launch_rowload<MF, NT, 10, VW>(st, a, step, beta, damp);
+}
+
+// Row-load kernels apply when rows are wave-uniform (S % 128 == 0) and the structure fits their
+// slots; returns the MAXB instance to use, or 0.
+int rowload_slots(const cvfes_system_t *sys) {
+    if (sys->n_smp % 128 != 0) return 0;
+    if (sys->max_row_blocks < 1 || sys->max_offdiag_contrib < 1 || sys->max_diag_contrib < 1) return 0;
+    if (sys->max_offdiag_contrib > 2) return 0;
+    for (int maxb : {7, 10})
+        if (sys->max_row_blocks <= maxb && sys->max_diag_contrib <= maxb - 1) return maxb;
+    return 0;
+}
+
+void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const StepArgs &a,
+                   int64_t step, bool beta, bool damp) {
+    const int maxb = rowload_slots(sys);
+    int64_t v = sys->variant;
+    // Measured on MI355X (102,400-node tube, 128 samples): the stored operator is fastest with the
+    // row-load kernel and non-temporal value loads; the matrix-free operator with the generic
+    // kernel (the row-load form needs too many registers for enough waves in flight).
+    if (v == CVFES_VARIANT_AUTO) v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
+    if (maxb && v != CVFES_VARIANT_GENERIC) {
+        const bool one = v == CVFES_VARIANT_ROWLOAD1 || v == CVFES_VARIANT_ROWLOAD1_NT;
+        const bool nt = v == CVFES_VARIANT_ROWLOAD_NT || v == CVFES_VARIANT_ROWLOAD1_NT;
+        if (mf) {
+            if (one) rowload_maxb<true, false, 1>(st, a, step, beta, damp, maxb);
+            else rowload_maxb<true, false, 2>(st, a, step, beta, damp, maxb);
+        } else if (one) {
+            if (nt) rowload_maxb<false, true, 1>(st, a, step, beta, damp, maxb);
+            else rowload_maxb<false, false, 1>(st, a, step, beta, damp, maxb);
+        } else {
+            if (nt) rowload_maxb<false, true, 2>(st, a, step, beta, damp, maxb);
+            else rowload_maxb<false, false, 2>(st, a, step, beta, damp, maxb);
+        }
+        return;
+    }
     const bool v2 = a.k.n_smp % 2 == 0;
     if (mf) {
         if (v2) step_u<true, 2>(st, a, step, beta, damp); else step_u<true, 1>(st, a, step, beta, damp);
@@ -393,6 +620,8 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     CVFES_REQUIRE(nsteps >= 0, "cvfes_step: nsteps < 0");
     CVFES_REQUIRE(dt > 0.0, "cvfes_step: dt must be positive");
     CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_step: lm_sstride must be 0 or 1");
+    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_ROWLOAD1_NT,
+                  "cvfes_step: unknown variant %lld", static_cast<long long>(sys->variant));
     hipStream_t st = static_cast<hipStream_t>(stream);
     StepArgs a{};
     a.k = row_args(sys);
@@ -411,7 +640,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
         a.u = cur;
         a.up = prev;
         a.out = nxt;
-        step_dispatch(st, mf, a, n, beta != 0.0, damp != 0.0);
+        step_dispatch(st, sys, mf, a, n, beta != 0.0, damp != 0.0);
         if (int rc = check_launch("k_step")) return rc;
         double *t = prev;
         prev = cur;

@@ -36,8 +36,8 @@ class LoadTables:
         f64 = torch.float64
         self.p = p.to(device, f64).contiguous() if isinstance(p, torch.Tensor) else _dev(np.atleast_1d(np.asarray(p, np.float64)), f64, device)
         self.n = int(self.p.numel())
-        self.ga = None if ga is None else _dev(np.asarray(ga, np.float64).reshape(-1, 3), f64, device)
-        self.gb = None if gb is None else _dev(np.asarray(gb, np.float64).reshape(-1, 3), f64, device)
+        self.ga = self._field(ga, device)
+        self.gb = self._field(gb, device)
         self.qa = None if qa is None else _dev(np.asarray(qa, np.float64).reshape(-1), f64, device)
         self.qb = None if qb is None else _dev(np.asarray(qb, np.float64).reshape(-1), f64, device)
         for q in (self.qa, self.qb):
@@ -45,6 +45,14 @@ class LoadTables:
                 raise ValueError("load tables must all have one entry per step")
         if (self.ga is None) != (self.qa is None) or (self.gb is None) != (self.qb is None):
             raise ValueError("a traction field needs its table and vice versa")
+
+    @staticmethod
+    def _field(g, device):
+        if g is None:
+            return None
+        if isinstance(g, torch.Tensor):
+            return g.to(device, torch.float64).reshape(-1, 3).contiguous()
+        return _dev(np.asarray(g, np.float64).reshape(-1, 3), torch.float64, device)
 
     def struct(self, offset=0):
         """``cvfes_load_t`` whose step 0 is entry ``offset`` of the tables."""
@@ -90,6 +98,14 @@ class MembraneSystem:
         self.nnzb = int(self.indices_h.size)
         maps = BlockMaps(elements, self.n_nodes, self.indptr_h, self.indices_h)
         self.maps = maps
+        # exact structure maxima for the row-load kernels
+        row_of = np.repeat(np.arange(self.n_nodes), np.diff(self.indptr_h))
+        nc = np.diff(maps.cptr.astype(np.int64))
+        diag = self.indices_h == row_of
+        self.max_row_blocks = int(np.diff(self.indptr_h).max())
+        self.max_offdiag_contrib = int(nc[~diag].max()) if np.any(~diag) else 1
+        self.max_diag_contrib = int(nc[diag].max()) if np.any(diag) else 1
+        self.variant = _lib.VARIANT_AUTO
         i32 = torch.int32
         self.nodes = _dev(nodes, torch.float64, dev)
         self.elems = _dev(elements, i32, dev)
@@ -172,7 +188,14 @@ class MembraneSystem:
             indptr=ptr(self.indptr), indices=ptr(self.indices), values=ptr(self.values),
             cptr=ptr(self.cptr), celem=ptr(self.celem), khat=ptr(self.khat), coef=ptr(self.coef),
             lm=ptr(self.lm), lm_sstride=self.thick_sstride, f1=ptr(self.f1), fixed=ptr(self.fixed),
-            bdy_u=self.bdy_u)
+            bdy_u=self.bdy_u, max_row_blocks=self.max_row_blocks,
+            max_offdiag_contrib=self.max_offdiag_contrib, max_diag_contrib=self.max_diag_contrib,
+            variant=self.variant)
+
+    def set_variant(self, variant):
+        """Select the step kernel variant (``auto``, ``generic``, ``rowload``, ``rowload_nt``)."""
+        self.variant = _lib.VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        self._sys = self._make_struct()
 
     @property
     def stream(self):

@@ -54,7 +54,22 @@ typedef struct cvfes_system {
     const double *f1;         /* [N][3] unit-pressure nodal load                               */
     const uint8_t *fixed;     /* [N] 1 = Dirichlet node (all 3 dofs), may be NULL              */
     double bdy_u;             /* prescribed displacement of fixed nodes                        */
+    /* Exact structure maxima (0 = unknown): blocks per row, element contributions per
+     * off-diagonal block, element contributions per diagonal block.  They enable the row kernels
+     * that issue all loads of a row up front; wrong values give wrong results. */
+    int64_t max_row_blocks;
+    int64_t max_offdiag_contrib;
+    int64_t max_diag_contrib;
+    int64_t variant;          /* CVFES_VARIANT_*                                               */
 } cvfes_system_t;
+
+/* Step kernel variants (cvfes_system_t.variant). */
+#define CVFES_VARIANT_AUTO 0       /* fastest measured variant the structure allows               */
+#define CVFES_VARIANT_GENERIC 1    /* block-by-block row walk, any structure and sample count      */
+#define CVFES_VARIANT_ROWLOAD 2    /* all loads of a row issued up front, two samples per lane     */
+#define CVFES_VARIANT_ROWLOAD_NT 3 /* as ROWLOAD, stored values streamed with non-temporal loads   */
+#define CVFES_VARIANT_ROWLOAD1 4   /* as ROWLOAD with one sample per lane (two waves per row)       */
+#define CVFES_VARIANT_ROWLOAD1_NT 5 /* as ROWLOAD1 with non-temporal value loads                   */
 
 /* Time-dependent nodal load: f_n = p[n] * f1 + qa[n] * ga + qb[n] * gb  (ga/gb may be NULL).
  * ga, gb are sample-independent [N][3] traction fields (e.g. two consecutive fluid wall-stress

@@ -320,20 +320,29 @@ def step(indptr, indices, K, LM, LHS, f, u, up, dt, beta=0.0, damp=0.0, bdy_dofs
 
 
 def run(indptr, indices, K, LM, f1, pressures, u0, du0, dt, beta=0.0, damp=0.0, bdy_dofs=None,
-        bdy_u=0.0, record=None):
-    """Run ``len(pressures)`` steps; step ``n`` uses the load ``pressures[n] * f1``.
+        bdy_u=0.0, record=None, extra_forces=None, p_init=None, extra_init=None):
+    """Run ``len(pressures)`` steps; step ``n`` uses the load ``pressures[n] * f1 (+ extra_forces[n])``.
 
-    Returns ``(u_N, u_{N-1}, snapshots)`` where ``snapshots[k]`` is ``u`` after ``k`` steps for each
-    ``k`` in ``record``.
+    The initial acceleration uses ``p_init * f1 + extra_init`` (default: the step-0 load).
+    ``extra_forces`` ``[nsteps, ndof]`` are EXTENSIONS standing for the fluid wall traction
+    (``physicsSolverSDampGPUs.py:384-390``).  Returns ``(u_N, u_{N-1}, snapshots)`` where
+    ``snapshots[k]`` is ``u`` after ``k`` steps for each ``k`` in ``record``.
     """
     pressures = np.asarray(pressures, float)
-    u, up, LHS = initial_state(indptr, indices, K, LM, pressures[0] * f1, u0, du0, dt, beta, damp)
+    p0 = pressures[0] if p_init is None else p_init
+    f0 = p0 * f1
+    if extra_init is not None:
+        f0 = f0 + extra_init
+    elif extra_forces is not None:
+        f0 = f0 + extra_forces[0]
+    u, up, LHS = initial_state(indptr, indices, K, LM, f0, u0, du0, dt, beta, damp)
     snaps = {}
     record = set(record or ())
     if 0 in record:
         snaps[0] = u.copy()
     for n, p in enumerate(pressures):
-        unew = step(indptr, indices, K, LM, LHS, p * f1, u, up, dt, beta, damp, bdy_dofs, bdy_u)
+        fn = p * f1 if extra_forces is None else p * f1 + extra_forces[n]
+        unew = step(indptr, indices, K, LM, LHS, fn, u, up, dt, beta, damp, bdy_dofs, bdy_u)
         up, u = u, unew
         if n + 1 in record:
             snaps[n + 1] = u.copy()

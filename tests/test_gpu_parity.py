@@ -87,6 +87,9 @@ def test_spmv_matches_oracle(case1, method):
     assert rel_err(y.reshape(-1, case1.S), yref) < 1e-13
 
 
+_ORACLE_CACHE = {}
+
+
 def _run_both(case, nsteps, method, beta=0.0, damp=0.0, dt=None, record=()):
     c = case.case
     dt = c.dt if dt is None else dt
@@ -100,10 +103,12 @@ def _run_both(case, nsteps, method, beta=0.0, damp=0.0, dt=None, record=()):
         sysm.step(p, dt, beta=beta, damp=damp, method=method, offset=done, nsteps=k - done)
         done = k
         out[k] = sysm.u.cpu().numpy().reshape(-1, case.S)
-    _, _, snaps = so.run(case.indptr, case.indices, case.K, case.LM, case.f1, p, u0, u0, dt,
-                         beta=beta, damp=damp, bdy_dofs=case.bdy_dofs, bdy_u=0.0,
-                         record=set(record) | {nsteps})
-    return out, snaps
+    key = (id(case), nsteps, beta, damp, dt, tuple(sorted(record)))
+    if key not in _ORACLE_CACHE:
+        _ORACLE_CACHE[key] = so.run(case.indptr, case.indices, case.K, case.LM, case.f1, p, u0, u0,
+                                    dt, beta=beta, damp=damp, bdy_dofs=case.bdy_dofs, bdy_u=0.0,
+                                    record=set(record) | {nsteps})[2]
+    return out, _ORACLE_CACHE[key]
 
 
 @pytest.mark.parametrize("method", ["stored", "matfree"])
@@ -132,3 +137,26 @@ def test_node_field_modulus_and_odd_sample_count(require_gpu):
     out, snaps = _run_both(c, 200, "matfree", record=(7,))
     for k, u in out.items():
         assert rel_err(u, snaps[k]) < REL_U
+
+
+@pytest.fixture(scope="module")
+def case128(require_gpu):
+    return Case("tube-64x32-s16", n_smp=128, seed=9)
+
+
+@pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt"])
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_kernel_variants_match_oracle(case128, method, variant):
+    """Every step-kernel variant at 128 samples (wave-uniform rows) against the oracle."""
+    case128.sys.set_variant(variant)
+    try:
+        out, snaps = _run_both(case128, 120, method, beta=1e-6, damp=30.0, dt=3e-6, record=(1, 40))
+    finally:
+        case128.sys.set_variant("auto")
+    for k, u in out.items():
+        assert rel_err(u, snaps[k]) < REL_U, f"step {k}"
+
+
+def test_structure_limits(case128):
+    s = case128.sys
+    assert (s.max_row_blocks, s.max_offdiag_contrib, s.max_diag_contrib) == (7, 2, 6)

@@ -1,0 +1,169 @@
+"""The drop-in ``GPUSolidSolver`` driven like CVFES drives it, against the CPU oracle (GPU).
+
+* ``TransientSolverGPU`` per-step loop (fluid time rounded to 4 decimals, ``solver.py:118``);
+* the fast ``Run`` path (exact step times);
+* fluid wall-traction input (snapshot series and constant stress) and the legacy module;
+* ``Save`` writes the per-sample displacement arrays.
+Bar: 1e-10 relative max-norm on the displacements.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import solid_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG = os.path.join(ROOT, "cvfes_amd", "cfg", "tube_64x32_s16.cfg")
+REL = 1e-10
+# 2^-18 s (< the 5.09e-6 s stability bound): a power of two, so `np.arange(endtime, step=dt)` in the
+# time loop (solver.py:75) gives exactly endtime/dt steps of exactly dt.
+DT = 2.0 ** -18
+
+
+def endtime(n):
+    return repr(n * DT)
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def write_cfg(tmp_path, **repl):
+    text = open(CFG).read()
+    for k, v in repl.items():
+        lines = []
+        for line in text.splitlines():
+            stripped = line.strip()
+            if stripped.startswith(k + " =") or stripped.startswith(k + "="):
+                indent = line[: len(line) - len(line.lstrip())]
+                line = f"{indent}{k} = {v}"
+            lines.append(line)
+        text = "\n".join(lines)
+    path = tmp_path / "case.cfg"
+    path.write_text(text)
+    return str(path)
+
+
+class OracleModel:
+    """Oracle assembly of the mesh a CVFES driver built."""
+
+    def __init__(self, mesh):
+        el, n, S = mesh.elementNodeIds, mesh.nNodes, mesh.nSmp
+        self.indptr, self.indices = so.sparse_info_loops(el, n)
+        _, gth = so.gauss_properties(mesh.vE, mesh.vthickness, el)
+        coefK = so.coef_k_element_E(gth, mesh.elmE)
+        _, self.K, self.f1, self.LM = so.assemble_reference(mesh.nodes, el, 1.0, so.d_matrix(mesh.v),
+                                                            coefK, gth, mesh.density, self.indptr,
+                                                            self.indices)
+        self.bdy_dofs = (3 * mesh.boundary[:, None] + np.arange(3)).ravel()
+
+    def run(self, pressures, dt, beta=0.0, damp=0.0, **kw):
+        n = self.indptr.size - 1
+        u0 = np.zeros(3 * n)
+        u, _, _ = so.run(self.indptr, self.indices, self.K, self.LM, self.f1, pressures, u0, u0, dt,
+                         beta=beta, damp=damp, bdy_dofs=self.bdy_dofs, **kw)
+        return u
+
+
+@pytest.fixture()
+def driver(require_gpu):
+    from cvfes_amd.cvfes import CVFES
+    return CVFES
+
+
+def test_transient_loop_matches_oracle(tmp_path, driver):
+    cfg = write_cfg(tmp_path, endtime=endtime(200), dt=repr(DT))
+    cv = driver().ReadInputFile(cfg)
+    solver = cv.Solve()
+    mesh = cv.meshes["wall"]
+    n_steps = solver.nTimeSteps
+    assert n_steps == 200
+    wf = solver.solidSolver.waveform
+    p = np.array([float(wf(round(solver.t[k], 4))) for k in range(n_steps)])
+    om = OracleModel(mesh)
+    u_ref = om.run(p, DT, p_init=float(wf(0.0)))
+    assert rel_err(solver.solidSolver.GetDisplacement(), u_ref) < REL
+
+
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_fast_run_matches_oracle_exact_times(tmp_path, driver, method):
+    cfg = write_cfg(tmp_path, endtime=endtime(300), dt=repr(DT), stiffness_method=method,
+                    beta="2.0e-7", Damp="20.0")
+    cv = driver().ReadInputFile(cfg)
+    solver = cv.Solve(fast=True)
+    mesh = cv.meshes["wall"]
+    n_steps = solver.nTimeSteps
+    wf = solver.solidSolver.waveform
+    assert n_steps == 300
+    p = wf(np.arange(n_steps) * DT)
+    om = OracleModel(mesh)
+    u_ref = om.run(p, DT, beta=2e-7, damp=20.0)
+    assert rel_err(solver.solidSolver.GetDisplacement(), u_ref) < REL
+
+
+@pytest.mark.parametrize("legacy", [False, True])
+def test_wall_traction_series(tmp_path, driver, legacy):
+    from cvfes_amd.loads import TractionSchedule
+    rng = np.random.default_rng(4)
+    n = 64 * 32
+    base = str(tmp_path / "wallStress")
+    for k in range(6):
+        np.save(f"{base}{k}.npy", rng.standard_normal((n, 3)) * 50.0)
+    cfg = write_cfg(tmp_path, endtime=endtime(150), dt=repr(DT), Damp="10.0", constant_T="2.0e-4",
+                    dt_f="1.2e-4")
+    text = open(cfg).read().replace("sample_num = 16", f"sample_num = 16\nexport_stress_filename = '{base}'")
+    open(cfg, "w").write(text)
+    from cvfes_amd.cvfes import CVFES
+    cv = CVFES().ReadInputFile(cfg)
+    if legacy:
+        from cvfes_amd.physicsSolverGPUs import GPUSolidSolver as Legacy
+        from cvfes_amd.solver import TransientSolver
+
+        class LegacyLoop(TransientSolver):
+            solid_class = Legacy
+        solver = LegacyLoop(cv.comm, cv.meshes, cv.cvConfig.solver)
+    else:
+        from cvfes_amd.solver import TransientSolverGPU
+        solver = TransientSolverGPU(cv.comm, cv.meshes, cv.cvConfig.solver)
+    solver.Solve(fast=True)
+    mesh = cv.meshes["wall"]
+    steps = solver.nTimeSteps
+    sched = TractionSchedule(base, False, 2.0e-4, 1.2e-4, n, mesh.boundary, legacy=legacy)
+    assert steps == 150
+    times = np.arange(steps) * DT
+    extra = []
+    for t in times:
+        _, ga, qa, gb, qb = sched.fields(t)
+        f = qa * ga + (0.0 if gb is None else qb * gb)
+        extra.append(f.ravel())
+    extra = np.array(extra)
+    wf = solver.solidSolver.waveform
+    om = OracleModel(mesh)
+    u_ref = om.run(wf(times), DT, damp=0.0 if legacy else 10.0, extra_forces=extra)
+    assert rel_err(solver.solidSolver.GetDisplacement(), u_ref) < REL
+
+
+def test_save_writes_per_sample_displacements(tmp_path, driver):
+    from cvfes_amd import vtkio
+    out = str(tmp_path / "disp.vtp")
+    cfg = write_cfg(tmp_path, endtime=endtime(50), dt=repr(DT), save_result_num="2",
+                    save_stress_filename=f"'{out}'")
+    cv = driver().ReadInputFile(cfg)
+    solver = cv.Solve()
+    path = str(tmp_path / "disp50.vtp")
+    assert os.path.exists(path) and os.path.exists(str(tmp_path / "disp25.vtp"))
+    back = vtkio.read_vtp(path)
+    u = solver.solidSolver.GetDisplacement()
+    assert len([k for k in back.point_data if k.startswith("u_")]) == 16
+    assert np.array_equal(back.point_data["u_003"].ravel(), u[:, 3])
+
+
+def test_refuses_without_library(monkeypatch, tmp_path):
+    from cvfes_amd import _lib
+    monkeypatch.setattr(_lib, "_LIB", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "missing.so")
+    with pytest.raises(_lib.CvfesError):
+        _lib.lib()

@@ -1,0 +1,220 @@
+"""Host-side logic on the CPU: configuration files, loads, VTP I/O, meshes, GMRF inputs, the C ABI."""
+import glob
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cvfes_amd.cvconfig import CVConfig, parse_cfg, read_cfg
+from cvfes_amd.gmrf import dual_laplacian_system, element_dual_edges, sample_element_E
+from cvfes_amd.loads import PressureWaveform, TractionSchedule
+from cvfes_amd.mesh import stenosed_radius, tube_geometry, tube_solid_mesh
+from cvfes_amd.coloring import mesh_coloring
+from cvfes_amd import vtkio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG_DIR = os.path.join(ROOT, "cvfes_amd", "cfg")
+
+
+# ---------------------------------------------------------------- configuration ---------------
+def test_parser_sections_lists_comments_bools():
+    text = """
+a = 1 # comment
+name = 'x # not a comment'
+lst = 'p.vtp','q.vtp'
+flag = yes
+[top]
+    k = 2.5
+    [[sub 1]]
+        name = s1
+        [[[deep]]]
+            v = off
+    [[sub 2]]
+        name = s2
+"""
+    c = parse_cfg(text)
+    assert c.as_int("a") == 1 and c["name"] == "x # not a comment"
+    assert c["lst"] == ["p.vtp", "q.vtp"] and c.as_bool("flag") is True
+    assert c["top"].as_float("k") == 2.5 and c["top"].sections == ["sub 1", "sub 2"]
+    assert c["top"]["sub 1"]["deep"].as_bool("v") is False
+    assert c["top"]["sub 1"].parent is c["top"]
+
+
+def test_example_cfgs_parse():
+    for f in sorted(glob.glob(os.path.join(CFG_DIR, "*.cfg"))):
+        cv = CVConfig(read_cfg(f))
+        assert cv.solver.method == "transient GPU"
+        assert cv.meshes["wall"].generator == "tube"
+        assert cv.equations["solid"].EModel == "gmrf_element"
+        assert cv.solver.pressure_waveform == "sine"
+    cv = CVConfig(read_cfg(os.path.join(CFG_DIR, "tube_256x400_s128.cfg")))
+    assert (cv.meshes["wall"].ntheta, cv.meshes["wall"].nz, cv.nSmp) == (256, 400, 128)
+    assert cv.solver.beta == 1e-4 and cv.solver.dt == 1.2e-8
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/cfg"), reason="reference cfgs not present")
+def test_reference_cfgs_parse():
+    files = sorted(glob.glob("/root/reference/cfg/**/*.cfg", recursive=True))
+    assert files
+    for f in files:
+        cv = CVConfig(read_cfg(f))
+        assert "wall" in cv.meshes and cv.solver.dt > 0
+        solid = cv.equations["solid"]
+        assert solid.E > 0 and 0 <= solid.v < 0.5 + 1e-12
+
+
+# ---------------------------------------------------------------- loads --------------------------
+def test_pressure_waveforms():
+    ramp = PressureWaveform("ramp", p0=100.0, ramp_T=0.2)
+    assert ramp(0.0) == 0.0 and math.isclose(float(ramp(0.1)), 50.0) and ramp(0.3) == 100.0
+    sine = PressureWaveform("sine", mean=1.33e4, amplitude=0.1, period=0.8)
+    assert math.isclose(float(sine(0.2)), 1.33e4 * 1.1) and math.isclose(float(sine(0.0)), 1.33e4)
+
+
+def test_traction_schedules(tmp_path):
+    n = 5
+    bdy = np.array([0, 4])
+    base = str(tmp_path / "trac")
+    for k in range(4):
+        np.save(f"{base}{k}.npy", np.full((n, 3), float(k + 1)))
+    np.save(f"{base}.npy", np.full((n, 3), 7.0))
+    # damped form: ramp on snapshot 0 until T, then snapshots from T
+    s = TractionSchedule(base, False, 0.2, 0.01, n, bdy)
+    key, ga, qa, gb, qb = s.fields(0.1)
+    assert gb is None and math.isclose(qa, 0.5) and ga[2, 0] == 1.0 and ga[0, 0] == 0.0
+    key, ga, qa, gb, qb = s.fields(0.2 + 0.0125)
+    f = qa * ga + qb * gb
+    assert math.isclose(f[1, 0], 2.0 + 0.25 * (3.0 - 2.0)) and f[4, 1] == 0.0
+    # legacy form: from a zero field at t = 0
+    s = TractionSchedule(base, False, 0.2, 0.01, n, bdy, legacy=True)
+    key, ga, qa, gb, qb = s.fields(0.005)
+    assert math.isclose((qa * ga + qb * gb)[1, 0], 0.5)
+    key, ga, qa, gb, qb = s.fields(0.0125)
+    assert math.isclose((qa * ga + qb * gb)[1, 0], 1.25)
+    # constant form
+    s = TractionSchedule(base, True, 0.2, 0.01, n, bdy)
+    key, ga, qa, gb, qb = s.fields(0.5)
+    assert qa == 1.0 and ga[1, 2] == 7.0
+
+
+# ---------------------------------------------------------------- VTP ---------------------------
+@pytest.mark.parametrize("fmt", ["binary", "ascii"])
+def test_vtp_round_trip(tmp_path, fmt):
+    nodes, el, _ = tube_geometry(6, 3, 0.3, 1.0)
+    u = np.random.default_rng(0).standard_normal((nodes.shape[0], 3))
+    pd = vtkio.PolyData(nodes, el, {"u_000": u, "GlobalNodeID": np.arange(nodes.shape[0], dtype=np.int64)},
+                        {"xx_000": np.arange(len(el), dtype=float)})
+    path = str(tmp_path / "m.vtp")
+    vtkio.write_vtp(path, pd, fmt)
+    back = vtkio.read_vtp(path)
+    assert np.array_equal(back.points, nodes) and np.array_equal(back.polys, el)
+    assert np.array_equal(back.point_data["u_000"], u)
+    assert np.array_equal(back.point_data["GlobalNodeID"], np.arange(nodes.shape[0]))
+    assert np.array_equal(back.cell_data["xx_000"], np.arange(len(el), dtype=float))
+
+
+def test_save_displacement_names(tmp_path):
+    nodes, el, _ = tube_geometry(5, 3, 0.3, 1.0)
+    u = np.zeros((2, nodes.shape[0], 3))
+    u[1, :, 2] = 1.0
+    path = vtkio.save_displacement(str(tmp_path / "disp.vtp"), 7, nodes, el, u)
+    assert path.endswith("disp7.vtp")
+    back = vtkio.read_vtp(path)
+    assert set(back.point_data) == {"u_000", "u_001"} and back.point_data["u_001"][3, 2] == 1.0
+
+
+# ---------------------------------------------------------------- meshes ------------------------
+def test_tube_orientation_counts_boundary():
+    nt, nz = 24, 7
+    nodes, el, bdy = tube_geometry(nt, nz, 0.4, 2.0)
+    assert nodes.shape == (nt * nz, 3) and el.shape == (2 * nt * (nz - 1), 3)
+    X = nodes[el]
+    n = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+    c = X.mean(axis=1)
+    assert np.all(np.einsum("ij,ij->i", n[:, :2], c[:, :2]) > 0)          # outward normals
+    assert np.allclose(nodes[bdy[:nt], 2], 0.0) and np.allclose(nodes[bdy[nt:], 2], 2.0)
+
+
+def test_stenosed_profile():
+    prof = stenosed_radius(1.2, 0.8, 12.0, severity=0.5)
+    assert math.isclose(float(prof(6.0)), 0.5 * 1.0) and math.isclose(float(prof(0.0)), 1.2, rel_tol=1e-6)
+    nodes, el, _ = tube_geometry(16, 50, 1.2, 12.0, prof)
+    r = np.hypot(nodes[:, 0], nodes[:, 1])
+    assert r.min() < 0.51 and r.max() <= 1.2 + 1e-12
+
+
+def test_solid_mesh_attributes_and_coloring():
+    m = tube_solid_mesh(8, 4, 3)
+    for attr in ("ndof", "nNodes", "nElements", "nodes", "elementNodeIds", "vE", "vthickness", "density",
+                 "v", "damp", "iniU", "iniDu", "bdyU", "boundary", "lclNNodes", "lclNodeIds",
+                 "lclElmNodeIds", "lclNCommNodes", "lclNSpecialHead", "lclBoundary"):
+        assert hasattr(m, attr), attr
+    groups = m.colorGroups
+    colors = m.elmColors
+    for g in groups:                                   # no two elements of a colour share a node
+        nodes_in = m.elementNodeIds[g].ravel()
+        assert nodes_in.size == np.unique(nodes_in).size
+    assert sum(len(g) for g in groups) == m.nElements and colors.max() < 23
+    with pytest.raises(RuntimeError):
+        mesh_coloring(m.elementNodeIds, m.nNodes, n_colors=2)
+
+
+# ---------------------------------------------------------------- GMRF --------------------------
+def test_dual_graph_edges():
+    nodes, el, _ = tube_geometry(6, 3, 0.3, 1.0)
+    pairs = element_dual_edges(el)
+    # each interior triangle edge is shared by 2 elements: interior edges = 3E/2 - boundary/2
+    assert pairs.shape[0] == (3 * len(el) - 2 * 6) // 2
+    A, B = dual_laplacian_system(el)
+    assert np.allclose(A.sum(axis=1), 1.0)
+
+
+def test_gmrf_covariance_matches_precision_inverse():
+    nodes, el, _ = tube_geometry(4, 3, 0.3, 1.0)           # 16 elements
+    A, _ = dual_laplacian_system(el)
+    g = sample_element_E(el, 40000, seed=11, cov=0.1)
+    cov = np.cov(g.z)
+    target = np.linalg.inv(g.tau * A.toarray())
+    assert np.abs(cov - target).max() < 0.05 * target.max()
+    assert math.isclose(float(np.mean(np.diag(target))), g.var, rel_tol=1e-10)
+    cv = g.E.std(axis=1) / g.E.mean(axis=1)
+    assert abs(cv.mean() - 0.1) < 0.01 and abs(g.E.mean() / 7e6 - 1.0) < 0.01
+
+
+# ---------------------------------------------------------------- C ABI -------------------------
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "cvfes_mi355x.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(cvfes_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from cvfes_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libcvfes_mi355x.so not built")
+    h = ctypes.CDLL(str(_lib.LIB_PATH))
+    names = header_functions()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(h, name), name
+    assert set(names) == set(_lib.EXPORTED)
+    assert h.cvfes_abi_version() == _lib.ABI_VERSION
+
+
+def test_abi_argument_errors_without_gpu():
+    """Argument validation runs on the host: null/invalid inputs return CVFES_ERR_ARG, no launch."""
+    import ctypes
+    from cvfes_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libcvfes_mi355x.so not built")
+    lib = _lib.lib()
+    assert lib.cvfes_assemble_bcsr(None, 10, 4, None, None, None, None, None) == -1
+    assert b"null pointer" in lib.cvfes_last_error()
+    assert lib.cvfes_bcsr_spmv(None, None, 0, None, None) == -1
+    sysd = _lib.CvfesSystem(n_nodes=4, n_smp=2, nnzb=4)
+    assert lib.cvfes_bcsr_spmv(None, ctypes.byref(sysd), 0, None, None) == -1
+    assert b"block structure" in lib.cvfes_last_error()
+    assert lib.cvfes_element_matrices(None, 3, None, None, 0.3, 5 / 6, None, None, None, None) == -1
+    assert lib.cvfes_element_matrices(None, 0, None, None, 0.3, 5 / 6, None, None, None, None) == 0

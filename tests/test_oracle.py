@@ -1,0 +1,381 @@
+"""Pins of the CPU oracle against facts that do not come from its own formulas (CPU only).
+
+* block structure: brute-force neighbour sets and the closed-form block count of the tube;
+* element stiffness: a textbook constant-strain triangle, the factorised block formula of the GPU
+  kernels, symmetry / positive semi-definiteness, rigid translation and in-plane rotation null
+  space, and the constant-strain patch test;
+* lumped mass and load: closed forms on the faceted tube;
+* products: dense matrix-vector products;
+* time stepping: closed-form solutions of the central-difference recurrence.
+Planted mistakes (in the oracle, on the CPU) show that the pins catch a dropped term, a swapped
+operand and a wrong sign.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cvfes_amd.mesh import tube_geometry
+from cvfes_amd.sparse import BlockMaps, SparseInfo, block_structure
+from oracle import solid_oracle as so
+
+
+def brute_force_structure(el, n):
+    nb = [set() for _ in range(n)]
+    for tri in el:
+        for a in tri:
+            nb[a].update(int(b) for b in tri)
+    indptr = np.cumsum([0] + [len(s) for s in nb])
+    indices = np.concatenate([sorted(s) for s in nb]).astype(np.int64)
+    return indptr, indices
+
+
+@pytest.mark.parametrize("nt,nz", [(3, 2), (4, 3), (7, 5), (16, 9)])
+def test_structure_matches_brute_force_and_closed_form(nt, nz):
+    nodes, el, _ = tube_geometry(nt, nz, 0.3, 1.0)
+    n = nodes.shape[0]
+    ip, ix = so.sparse_info_loops(el, n)
+    bip, bix = brute_force_structure(el, n)
+    assert np.array_equal(ip, bip) and np.array_equal(ix, bix)
+    # product host code (vectorised) equals the oracle bit for bit
+    pip, pix = block_structure(el, n)
+    assert pip.dtype == np.int64 and np.array_equal(pip, ip) and np.array_equal(pix, ix)
+    # nnzb = N + 2 * edges, edges = ntheta * (3 nz - 2) for the triangulated tube (ntheta > 3)
+    if nt > 3:
+        assert ix.size == n + 2 * nt * (3 * nz - 2)
+
+
+def test_structure_ragged_and_isolated_nodes():
+    # a node touched by no element keeps an empty row (indptr repeats), as sparse.py:49-53 does
+    el = np.array([[0, 1, 3], [1, 4, 3]])
+    ip, ix = so.sparse_info_loops(el, 6)
+    assert list(np.diff(ip)) == [3, 4, 0, 4, 3, 0]
+    pip, pix = block_structure(el, 6)
+    assert np.array_equal(pip, ip) and np.array_equal(pix, ix)
+
+
+def test_block_maps_are_consistent():
+    nodes, el, _ = tube_geometry(8, 4, 0.3, 1.0)
+    n = nodes.shape[0]
+    ip, ix = block_structure(el, n)
+    m = BlockMaps(el, n, ip, ix)
+    rows = np.repeat(np.arange(n), np.diff(ip))
+    for e in (0, 5, len(el) - 1):
+        for a in range(3):
+            for b in range(3):
+                p = m.block_of[9 * e + 3 * a + b]
+                assert rows[p] == el[e, a] and ix[p] == el[e, b]
+                c = m.cpos[9 * e + 3 * a + b]
+                assert m.cptr[p] <= c < m.cptr[p + 1] and m.celem[c] == e and m.cab[c] == 3 * a + b
+    assert np.all(np.diff(m.celem[m.cptr[3]:m.cptr[4]]) >= 0)       # element order inside a block
+    assert m.nptr[-1] == 3 * len(el)
+
+
+class _SI:
+    def __init__(self, el, n):
+        self.elementNodeIds = el
+        self.nNodes = n
+
+
+def test_sparseinfo_interface():
+    nodes, el, _ = tube_geometry(5, 3, 0.3, 1.0)
+    si = SparseInfo(_SI(el, nodes.shape[0]), 3)
+    assert si.length == si.indices.size and si.ndof == 3 * nodes.shape[0]
+    assert si.New().shape == (si.length, 3, 3)
+
+
+# ------------------------------------------------------------------------------------------------
+# element stiffness
+# ------------------------------------------------------------------------------------------------
+def one_element_K(X, v=0.0, coef=1.0):
+    """Assemble a single triangle through the oracle and return its 9x9 matrix."""
+    el = np.array([[0, 1, 2]])
+    ip, ix = so.sparse_info_loops(el, 3)
+    coefK = np.array([[coef]])
+    gth = np.ones((1, 1, 3))
+    _, K, _, _ = so.assemble_reference(np.asarray(X, float), el, 1.0, so.d_matrix(v), coefK, gth,
+                                       1.0, ip, ix)
+    out = np.zeros((9, 9))
+    for a in range(3):
+        for p in range(ip[a], ip[a + 1]):
+            out[3 * a:3 * a + 3, 3 * ix[p]:3 * ix[p] + 3] = K[p, 0]
+    return out
+
+
+# Textbook constant-strain triangle, nodes (0,0), (1,0), (0,1), E = t = 1, nu = 0, in-plane dofs
+# (u1, v1, u2, v2, u3, v3): K = 1/4 * this integer matrix (e.g. Logan, "A First Course in the
+# Finite Element Method", CST example with nu = 0).
+CST_TEXTBOOK_X4 = np.array([
+    [3, 1, -2, -1, -1, 0],
+    [1, 3, 0, -1, -1, -2],
+    [-2, 0, 2, 0, 0, 0],
+    [-1, -1, 0, 1, 1, 0],
+    [-1, -1, 0, 1, 1, 0],
+    [0, -2, 0, 0, 0, 2],
+], dtype=float)
+
+
+def test_textbook_right_triangle():
+    X = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    K = one_element_K(X, v=0.0)
+    inplane = [0, 1, 3, 4, 6, 7]
+    assert np.allclose(K[np.ix_(inplane, inplane)], CST_TEXTBOOK_X4 / 4.0, atol=1e-14)
+    # transverse (z) rows: k/2 (b_a b_b + c_a c_b) / (4A) with k = 5/6: node 1 diagonal = 5/12 * 2 / 2
+    zz = K[np.ix_([2, 5, 8], [2, 5, 8])]
+    expected = (5.0 / 12.0) * np.array([[2, -1, -1], [-1, 1, 0], [-1, 0, 1]]) / 2.0
+    assert np.allclose(zz, expected, atol=1e-14)
+
+
+def test_closed_form_equals_cython_order_on_random_triangles():
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((30, 3))
+    el = np.array([[3 * i, 3 * i + 1, 3 * i + 2] for i in range(10)])
+    kh, area, nrm = so.element_blocks_closed_form(X, el, 0.37)
+    for e in range(10):
+        K = one_element_K(X[el[e]], v=0.37)
+        assert np.allclose(kh[e], K, rtol=0, atol=1e-12 * np.abs(K).max())
+        assert area[e] > 0
+        a, b = X[el[e, 1]] - X[el[e, 0]], X[el[e, 2]] - X[el[e, 0]]
+        n = np.cross(a, b)
+        assert np.allclose(nrm[e], n / np.linalg.norm(n))
+        assert np.isclose(area[e], 0.5 * np.linalg.norm(n))
+
+
+def test_symmetry_psd_and_rigid_null_space():
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        X = rng.standard_normal((3, 3))
+        K = one_element_K(X, v=0.3, coef=2.5)
+        assert np.allclose(K, K.T, atol=1e-12 * np.abs(K).max())
+        assert np.linalg.eigvalsh(K).min() > -1e-10 * np.abs(K).max()
+        t = rng.standard_normal(3)
+        assert np.abs(K @ np.tile(t, 3)).max() < 1e-12 * np.abs(K).max()
+        # infinitesimal rotation about the element normal: u = w x (X - X0)
+        n = np.cross(X[1] - X[0], X[2] - X[0])
+        n /= np.linalg.norm(n)
+        rot = np.concatenate([np.cross(n, X[a] - X[0]) for a in range(3)])
+        assert np.abs(K @ rot).max() < 1e-11 * np.abs(K).max() * np.abs(X).max()
+
+
+def test_patch_test_constant_strain():
+    # flat structured patch in the plane z = 0.3 x (tilted), interior nodes see no force under a
+    # linear (constant-strain) in-plane displacement field
+    nx, ny = 5, 4
+    xs, ys = np.meshgrid(np.linspace(0, 1, nx), np.linspace(0, 0.7, ny), indexing="ij")
+    P = np.stack([xs.ravel(), ys.ravel(), 0.3 * xs.ravel()], axis=1)
+    idx = np.arange(nx * ny).reshape(nx, ny)
+    el = []
+    for i in range(nx - 1):
+        for j in range(ny - 1):
+            el += [[idx[i, j], idx[i + 1, j], idx[i + 1, j + 1]], [idx[i, j], idx[i + 1, j + 1], idx[i, j + 1]]]
+    el = np.array(el)
+    n = P.shape[0]
+    ip, ix = so.sparse_info_loops(el, n)
+    coefK = np.full((len(el), 1), 3.0)
+    gth = np.ones((len(el), 1, 3))
+    _, K, _, _ = so.assemble_reference(P, el, 1.0, so.d_matrix(0.25), coefK, gth, 1.0, ip, ix)
+    A = np.array([[0.01, 0.02, 0.0], [-0.03, 0.015, 0.0], [0.003, 0.006, 0.0]])
+    u = (P @ A.T).reshape(-1, 1)
+    r = so.multiply_by_vector(ip, ix, K, u).reshape(n, 3)
+    interior = [idx[i, j] for i in range(1, nx - 1) for j in range(1, ny - 1)]
+    scale = np.abs(r).max()
+    assert scale > 0
+    assert np.abs(r[interior]).max() < 1e-12 * scale
+
+
+def test_planted_mistakes_are_caught(monkeypatch):
+    X = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    inplane = [0, 1, 3, 4, 6, 7]
+    good = so.d_matrix
+
+    def wrong_sign(v, k=5.0 / 6.0):          # nu with the wrong sign in D
+        return good(-v if v else 0.2, k)
+
+    def dropped_shear(v, k=5.0 / 6.0):       # in-plane shear term dropped
+        D = good(v, k)
+        D[2, 2] = 0.0
+        return D
+
+    for bad in (wrong_sign, dropped_shear):
+        monkeypatch.setattr(so, "d_matrix", bad)
+        K = one_element_K(X, v=0.0)
+        assert not np.allclose(K[np.ix_(inplane, inplane)], CST_TEXTBOOK_X4 / 4.0, atol=1e-6)
+    monkeypatch.setattr(so, "d_matrix", good)
+    # the frame applied transposed (T k T^T instead of T^T k T) breaks the rotation null space
+    Xt = np.array([[0.1, 0.2, 0.3], [1.2, -0.1, 0.4], [0.3, 0.9, -0.5]])
+    T = so.coordinate_transformation(Xt, np.array([[0, 1, 2]]))
+    kh, _, _ = so.element_blocks_closed_form(Xt, np.array([[0, 1, 2]]), 0.3)
+    Kbad = kh[0].copy()
+    Tb = np.kron(np.eye(3), T[0])
+    Kbad = Tb @ (Tb @ Kbad @ Tb.T) @ Tb.T          # = T k T^T with k = T K T^T
+    n = T[0, 2]
+    rot = np.concatenate([np.cross(n, Xt[a] - Xt[0]) for a in range(3)])
+    assert np.abs(kh[0] @ rot).max() < 1e-10
+    assert np.abs(Kbad @ rot).max() > 1e-6
+
+
+# ------------------------------------------------------------------------------------------------
+# mass, load, properties
+# ------------------------------------------------------------------------------------------------
+def tube_assembly(nt=12, nz=6, R=0.4, L=1.5, S=2, h=0.07, rho=1.3):
+    nodes, el, bdy = tube_geometry(nt, nz, R, L)
+    n = nodes.shape[0]
+    ip, ix = so.sparse_info_loops(el, n)
+    vE = np.full((n, S), 7e6)
+    vh = np.full((n, S), h)
+    gE, gth = so.gauss_properties(vE, vh, el)
+    coefK = so.coef_k(gth, gE)
+    M, K, f, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(0.45), coefK, gth, rho, ip, ix)
+    return nodes, el, bdy, ip, ix, M, K, f, LM, coefK
+
+
+def test_lumped_mass_total_is_one_third_of_textbook():
+    nt, nz, R, L, h, rho = 12, 6, 0.4, 1.5, 0.07, 1.3
+    nodes, el, bdy, ip, ix, M, K, f, LM, coefK = tube_assembly(nt, nz, R, L, 2, h, rho)
+    dz = L / (nz - 1)
+    facet_area = R * math.sin(math.pi / nt) * dz              # 1/2 * chord * dz
+    total_area = 2 * nt * (nz - 1) * facet_area
+    # every dof of a node carries the node's mass; sum over nodes = rho h A / 3
+    node_mass = LM[0::3]
+    assert np.allclose(LM[1::3], node_mass) and np.allclose(LM[2::3], node_mass)
+    assert np.allclose(node_mass.sum(axis=0), rho * h * total_area / 3.0, rtol=1e-13)
+    # interior node: 6 incident facets, rho A h / 9 each
+    a = nt * 2 + 3
+    assert np.allclose(node_mass[a], 6 * rho * facet_area * h / 9.0, rtol=1e-13)
+
+
+def test_unit_pressure_load_closed_form():
+    nt, nz, R, L = 12, 6, 0.4, 1.5
+    nodes, el, bdy, ip, ix, M, K, f, LM, coefK = tube_assembly(nt, nz, R, L)
+    F = f.reshape(-1, 3)
+    assert np.abs(F.sum(axis=0)).max() < 1e-14
+    dz = L / (nz - 1)
+    a = nt * 2 + 5                                            # interior node
+    rhat = nodes[a, :2] / np.linalg.norm(nodes[a, :2])
+    expected = R * math.sin(2 * math.pi / nt) * dz
+    assert np.allclose(F[a, :2], expected * rhat, rtol=1e-12) and abs(F[a, 2]) < 1e-15
+    assert np.all(np.einsum("ij,ij->i", F[:, :2], nodes[:, :2]) > 0)      # outward
+
+
+def test_gauss_properties_and_coef():
+    rng = np.random.default_rng(0)
+    nodes, el, _ = tube_geometry(6, 3, 0.3, 1.0)
+    n = nodes.shape[0]
+    vE = rng.uniform(1, 2, (n, 3))
+    vh = np.full((n, 3), 0.07)
+    gE, gth = so.gauss_properties(vE, vh, el)
+    # edge-midpoint values of node fields
+    assert np.allclose(gE[:, :, 0], 0.5 * (vE[el[:, 0]] + vE[el[:, 1]]))
+    assert np.allclose(gE[:, :, 2], 0.5 * (vE[el[:, 1]] + vE[el[:, 2]]))
+    coefK = so.coef_k(gth, gE)
+    assert np.allclose(coefK, 0.07 * vE[el].mean(axis=1), rtol=1e-14)
+    assert np.allclose(so.coef_k_element_E(gth, np.full((len(el), 3), 5.0)), 0.35, rtol=1e-14)
+
+
+def test_spmv_equals_dense():
+    nodes, el, bdy, ip, ix, M, K, f, LM, coefK = tube_assembly(7, 4, S=3)
+    n = nodes.shape[0]
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((3 * n, 3))
+    y = so.multiply_by_vector(ip, ix, K, x)
+    for s in range(3):
+        dense = np.zeros((3 * n, 3 * n))
+        for a in range(n):
+            for p in range(ip[a], ip[a + 1]):
+                dense[3 * a:3 * a + 3, 3 * ix[p]:3 * ix[p] + 3] = K[p, s]
+        assert np.allclose(y[:, s], dense @ x[:, s], rtol=1e-13, atol=1e-13 * np.abs(y).max())
+    y1 = so.multiply_by_1d_vector(ip, ix, K, x[:, 0])
+    assert np.allclose(y1[:, 1], so.multiply_by_vector(ip, ix, K, np.repeat(x[:, :1], 3, 1))[:, 1])
+
+
+# ------------------------------------------------------------------------------------------------
+# time stepping: closed forms of the recurrence
+# ------------------------------------------------------------------------------------------------
+def sdof_system(k, m):
+    ip = np.array([0, 1])
+    ix = np.array([0])
+    K = np.zeros((1, 1, 3, 3))
+    K[0, 0] = k * np.eye(3)
+    LM = np.full((3, 1), m)
+    return ip, ix, K, LM
+
+
+def closed_form_recurrence(lam, h, beta, c, u0, n):
+    """u_{k+1} = (2 - h^2 lam - h beta lam - h c) u_k - (1 - h beta lam - h c) u_{k-1} from its
+    characteristic roots, started at u_0 = u0, u_{-1} = u0 (1 - h^2 lam / 2)."""
+    a1 = 2 - h * h * lam - h * beta * lam - h * c
+    a0 = -(1 - h * beta * lam - h * c)
+    r = np.roots([1.0, -a1, -a0]).astype(complex)
+    um1 = u0 * (1 - 0.5 * h * h * lam)
+    A = np.array([[1, 1], [1 / r[0], 1 / r[1]]])
+    cc = np.linalg.solve(A, np.array([u0, um1], dtype=complex))
+    return float(np.real(cc[0] * r[0] ** n + cc[1] * r[1] ** n))
+
+
+@pytest.mark.parametrize("beta,damp", [(0.0, 0.0), (2e-4, 0.0), (0.0, 30.0), (1e-4, 15.0)])
+def test_single_dof_matches_closed_form(beta, damp):
+    k, m, h, n = 4.0e6, 2.0, 5e-4, 400
+    ip, ix, K, LM = sdof_system(k, m)
+    u0 = np.array([1e-3, -2e-3, 0.5e-3])
+    u, up, _ = so.run(ip, ix, K, LM, np.zeros(3), np.zeros(n), u0, np.zeros(3), h, beta=beta,
+                      damp=damp)
+    lam = k / m
+    for d in range(3):
+        assert math.isclose(u[d, 0], closed_form_recurrence(lam, h, beta, damp / m, u0[d], n),
+                            rel_tol=1e-9, abs_tol=1e-15)
+
+
+def test_undamped_sdof_is_cosine():
+    k, m, h, n = 1.0e6, 1.0, 1e-3, 321
+    ip, ix, K, LM = sdof_system(k, m)
+    u0 = np.array([1.0, 0.0, 0.0])
+    u, _, _ = so.run(ip, ix, K, LM, np.zeros(3), np.zeros(n), u0, np.zeros(3), h)
+    omega = math.acos(1 - h * h * k / m / 2)
+    assert math.isclose(u[0, 0], math.cos(n * omega), rel_tol=0, abs_tol=1e-10)
+
+
+def test_free_flight_under_constant_load():
+    # zero stiffness: central differences are exact for u = 1/2 a t^2
+    nodes, el, bdy = tube_geometry(8, 4, 0.3, 1.0)
+    n = nodes.shape[0]
+    ip, ix = so.sparse_info_loops(el, n)
+    gth = np.full((len(el), 2, 3), 0.07)
+    M, K, f, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(0.5), np.zeros((len(el), 2)),
+                                        gth, 1.0, ip, ix)
+    h, steps, p = 1e-5, 37, 1.33e4
+    u, _, _ = so.run(ip, ix, K, LM, f, np.full(steps, p), np.zeros(3 * n), np.zeros(3 * n), h)
+    acc = p * f[:, None] / LM
+    assert np.allclose(u, 0.5 * acc * (steps * h) ** 2, rtol=1e-10, atol=1e-18)
+
+
+def test_boundary_values_and_planted_step_mistake(monkeypatch):
+    nodes, el, bdy, ip, ix, M, K, f, LM, coefK = tube_assembly(8, 5, S=2)
+    n = nodes.shape[0]
+    bd = (3 * bdy[:, None] + np.arange(3)).ravel()
+    u, up, _ = so.run(ip, ix, K, LM, f, np.full(20, 1e4), np.zeros(3 * n), np.zeros(3 * n), 1e-7,
+                      bdy_dofs=bd, bdy_u=0.0)
+    assert np.all(u[bd] == 0.0) and np.abs(u).max() > 0
+    # planting a sign error in the stiffness term breaks the single-dof closed form
+    ipp, ixx, Ks, LMs = sdof_system(4e6, 2.0)
+    good = so.multiply_by_vector
+    monkeypatch.setattr(so, "multiply_by_vector", lambda *a: -good(*a))
+    ub, _, _ = so.run(ipp, ixx, Ks, LMs, np.zeros(3), np.zeros(100), np.array([1e-3, 0, 0]),
+                      np.zeros(3), 5e-4)
+    assert not math.isclose(ub[0, 0], closed_form_recurrence(2e6, 5e-4, 0, 0, 1e-3, 100), rel_tol=1e-3)
+
+
+# ------------------------------------------------------------------------------------------------
+# C restatement agrees with the NumPy restatement
+# ------------------------------------------------------------------------------------------------
+def test_c_oracle_matches_numpy_oracle():
+    from oracle import cpu_oracle as co
+    nodes, el, bdy, ip, ix, M, K, f, LM, coefK = tube_assembly(10, 6, S=4)
+    Kc = co.assemble_K(nodes, el, so.d_matrix(0.45), coefK, ip, ix, threads=2)
+    assert np.allclose(Kc, K, rtol=0, atol=1e-13 * np.abs(K).max())
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal((3 * nodes.shape[0], 4)) * 1e-4
+    up = u + rng.standard_normal(u.shape) * 1e-6
+    bd = (3 * bdy[:, None] + np.arange(3)).ravel()
+    a = so.step(ip, ix, K, LM, LM, 1e4 * f, u, up, 1e-7, beta=1e-5, damp=3.0, bdy_dofs=bd)
+    b = co.step(ip, ix, K, LM, LM, f, 1e4, u, up, 1e-7, beta=1e-5, damp=3.0, bdy_dofs=bd, threads=2)
+    assert np.allclose(a, b, rtol=0, atol=1e-14 * np.abs(a).max())
