@@ -55,12 +55,20 @@ class CvfesLoad(ctypes.Structure):
     _fields_ = [("p", _vp), ("ga", _vp), ("qa", _vp), ("gb", _vp), ("qb", _vp)]
 
 
+class CvfesBatchMesh(ctypes.Structure):
+    """Mirror of ``cvfes_batch_mesh_t``."""
+
+    _fields_ = [("sys", CvfesSystem), ("method", _i64), ("state", _vp * 3), ("load", CvfesLoad),
+                ("dt", _f64), ("beta", _f64), ("damp", _f64), ("task0", _i64), ("n_chunks", _i64),
+                ("vw", _i64)]
+
+
 _SIGNATURES = {
     "cvfes_abi_version": (_int, []),
     "cvfes_last_error": (ctypes.c_char_p, []),
     "cvfes_stream_sync": (_int, [_vp]),
     "cvfes_element_matrices": (_int, [_vp, _i64, _vp, _vp, _f64, _f64, _vp, _vp, _vp, _vp]),
-    "cvfes_element_coef": (_int, [_vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "cvfes_element_coef": (_int, [_vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _int, _vp]),
     "cvfes_lumped_mass_load": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                       _f64, _f64, _vp, _vp]),
     "cvfes_assemble_bcsr": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
@@ -69,6 +77,18 @@ _SIGNATURES = {
                                    ctypes.POINTER(CvfesLoad), _f64, _f64, _f64, _vp, _vp, _vp]),
     "cvfes_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, ctypes.POINTER(_vp),
                           ctypes.POINTER(CvfesLoad), _i64, _f64, _f64, _f64]),
+    "cvfes_adjoint_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp, _vp, _vp, _vp,
+                                  _vp, _f64, _f64, _f64, _int]),
+    "cvfes_grad_accum": (_int, [_vp, ctypes.POINTER(CvfesSystem), _vp, _vp, _vp, _vp, _vp, _f64,
+                                _f64, _f64, _vp]),
+    "cvfes_batch_plan": (_int, [_vp, ctypes.POINTER(CvfesBatchMesh), _int, _vp,
+                                ctypes.POINTER(_i64)]),
+    "cvfes_batch_step": (_int, [_vp, _vp, _int, _i64, _int, _vp, _i64, _i64]),
+    "cvfes_counter_add": (_int, [_vp, _vp, _i64]),
+    "cvfes_batch_graph_create": (_int, [_vp, _int, _i64, _vp, _i64, _int, _i64,
+                                        ctypes.POINTER(_vp)]),
+    "cvfes_graph_launch": (_int, [_vp, _vp]),
+    "cvfes_graph_destroy": (_int, [_vp]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -1,0 +1,186 @@
+"""Gradient of a displacement-misfit loss with respect to per-element log Young's modulus.
+
+``AdjointProblem`` runs the forward ensemble step with checkpoints every ``checkpoint_every``
+steps, evaluates the per-sample loss
+
+    L_s = 1/2 sum_{k in obs} || u_s(t_k) - u_obs,s(t_k) ||^2,
+
+and runs the reverse sweep: for each segment between checkpoints the forward states are recomputed
+from the checkpoint, then ``cvfes_adjoint_step`` (the transposed step) and ``cvfes_grad_accum``
+(the element-wise derivative of the stiffness) run backwards through the segment.  Memory is one
+pair of states per checkpoint plus one segment of states.  ``EnsembleLoss`` wraps it as a
+``torch.autograd.Function`` of ``theta = log E [E, S]``.
+
+All device work is in the native kernels; torch provides the buffers, the per-sample loss
+reduction and the autograd plumbing.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import check, ptr
+from .system import LoadTables, MembraneSystem
+
+__all__ = ["AdjointProblem", "EnsembleLoss"]
+
+
+class AdjointProblem:
+    """Forward + adjoint of ``nsteps`` steps on a :class:`MembraneSystem`.
+
+    Parameters
+    ----------
+    system : assembled system (its ``coef``/values are replaced by :meth:`set_theta`).
+    load : :class:`LoadTables` with at least ``nsteps`` entries (entry ``n`` = load of step ``n``).
+    nsteps, dt, beta, damp, method : as for the forward step.
+    obs_steps : sorted step indices ``k`` (1..nsteps) at which ``u_k`` is observed.
+    u_obs : device tensor ``[len(obs_steps), N, 3, S]`` of observations (``None`` = zero).
+    checkpoint_every : checkpoint interval ``C`` (1 = keep every state, no recomputation).
+    """
+
+    def __init__(self, system: MembraneSystem, load: LoadTables, nsteps, dt, beta=0.0, damp=0.0,
+                 method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None):
+        self.sys = system
+        self.load = load
+        self.nsteps = int(nsteps)
+        self.dt, self.beta, self.damp = float(dt), float(beta), float(damp)
+        self.method = method
+        self.obs_steps = [int(k) for k in obs_steps]
+        if any(k < 1 or k > self.nsteps for k in self.obs_steps):
+            raise ValueError("observation steps must lie in 1..nsteps")
+        self.obs_index = {k: i for i, k in enumerate(self.obs_steps)}
+        self.u_obs = u_obs
+        self.C = max(1, int(checkpoint_every))
+        N = system.n_nodes
+        self.u0 = np.zeros(3 * N) if u0 is None else np.asarray(u0, np.float64)
+        self.du0 = np.zeros(3 * N) if du0 is None else np.asarray(du0, np.float64)
+        self.checkpoints = {}
+        self.stats = {}
+
+    # ------------------------------------------------------------------------------------------
+    def set_theta(self, theta):
+        self.sys.set_log_modulus(theta)
+
+    def _obs(self, k):
+        return None if self.u_obs is None else self.u_obs[self.obs_index[k]]
+
+    def _loss_term(self, u, k):
+        ref = self._obs(k)
+        r = u if ref is None else u - ref
+        return 0.5 * (r * r).sum(dim=(0, 1))
+
+    def _start(self):
+        self.sys.initial_state(self.u0, self.du0, self.load, self.dt, beta=self.beta,
+                               damp=self.damp, method=self.method)
+
+    def _advance(self, offset, n):
+        if n > 0:
+            self.sys.step(self.load, self.dt, beta=self.beta, damp=self.damp, method=self.method,
+                          offset=offset, nsteps=n)
+
+    def forward(self, record=False):
+        """Run the forward pass with checkpoints; returns the per-sample loss ``[S]``.
+
+        ``record=True`` also keeps the observed states in ``self.obs_states`` (in ``obs_steps``
+        order).
+        """
+        sysm = self.sys
+        self._start()
+        self.checkpoints = {}
+        self.obs_states = []
+        loss = torch.zeros(sysm.n_smp, dtype=torch.float64, device=sysm.device)
+        marks = sorted(set(range(0, self.nsteps, self.C)) | set(self.obs_steps) | {self.nsteps})
+        step = 0
+        for mark in marks:
+            self._advance(step, mark - step)
+            step = mark
+            if mark % self.C == 0 and mark < self.nsteps:
+                self.checkpoints[mark] = (sysm.u.clone(), sysm.up.clone())
+            if mark in self.obs_index:
+                loss += self._loss_term(sysm.u, mark)
+                if record:
+                    self.obs_states.append(sysm.u.clone())
+        self.u_final = sysm.u.clone()
+        return loss
+
+    # ------------------------------------------------------------------------------------------
+    def _adjoint_call(self, mu1, mu2, out, obs_u=None, obs_ref=None, w=None, project=1):
+        sysm = self.sys
+        check(sysm.lib.cvfes_adjoint_step(sysm.stream, ctypes.byref(sysm._sys),
+                                          sysm.method_id(self.method), ptr(mu1), ptr(mu2), ptr(out),
+                                          ptr(obs_u), ptr(obs_ref), ptr(w), self.dt, self.beta,
+                                          self.damp, project))
+
+    def _grad_call(self, mu, xc, xp, nu_scale, grad):
+        sysm = self.sys
+        check(sysm.lib.cvfes_grad_accum(sysm.stream, ctypes.byref(sysm._sys), ptr(sysm.elems),
+                                        ptr(sysm.cpos), ptr(mu), ptr(xc), ptr(xp), self.dt,
+                                        self.beta, nu_scale, ptr(grad)))
+
+    def backward(self, grad_out=None):
+        """Reverse sweep; returns ``dL/dtheta [E, S]`` for the loss weighted by ``grad_out [S]``."""
+        sysm = self.sys
+        dev = sysm.device
+        S = sysm.n_smp
+        w = (torch.ones(S, dtype=torch.float64, device=dev) if grad_out is None
+             else grad_out.to(dev, torch.float64).reshape(S).contiguous())
+        shape = sysm.state_shape()
+        zeros = torch.zeros(shape, dtype=torch.float64, device=dev)
+        mu = [torch.zeros(shape, dtype=torch.float64, device=dev) for _ in range(3)]
+        grad = torch.zeros((sysm.n_elms, S), dtype=torch.float64, device=dev)
+        N = self.nsteps
+        # mu_N = P g_N
+        if N in self.obs_index:
+            self._adjoint_call(zeros, zeros, mu[0], self.u_final, self._obs(N), w)
+        mu_n1, mu_n2, spare = mu[0], zeros.clone(), mu[1]    # mu_{n+1}, mu_{n+2}
+        seg_starts = sorted(self.checkpoints, reverse=True)
+        seg_bufs = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]
+        for s0 in seg_starts:
+            s1 = min(s0 + self.C, N)
+            # recompute u_{s0+1} .. u_{s1-1} from the checkpoint (u_{s0}, u_{s0-1}), writing each new
+            # state straight into its segment buffer
+            u_s0, u_sm1 = self.checkpoints[s0]
+            states = {s0 - 1: u_sm1, s0: u_s0}
+            for i, n in enumerate(range(s0, s1 - 1)):
+                sysm.state = [states[n], states[n - 1], seg_bufs[i]]
+                self._advance(n, 1)
+                states[n + 1] = seg_bufs[i]
+            for n in range(s1 - 1, s0 - 1, -1):
+                # gradient of step n: mu_{n+1}, x_n = u_n + (beta/dt)(u_n - u_{n-1})
+                self._grad_call(mu_n1, states[n], states[n - 1] if self.beta != 0.0 else None,
+                                self.dt * self.dt, grad)
+                if n == 0:
+                    break
+                obs_u = states[n] if n in self.obs_index else None
+                self._adjoint_call(mu_n1, mu_n2, spare, obs_u,
+                                   self._obs(n) if obs_u is not None else None,
+                                   w if obs_u is not None else None)
+                mu_n2, mu_n1, spare = mu_n1, spare, mu_n2
+            del states
+        sysm.state = None
+        # initial state: u_{-1} depends on K through (u0 + beta du0)
+        x0 = self.u0 + self.beta * self.du0
+        if np.any(x0 != 0.0):
+            lam_m1 = torch.empty(shape, dtype=torch.float64, device=dev)
+            self._adjoint_call(zeros, mu_n1, lam_m1, project=0)
+            x0_d = torch.as_tensor(np.repeat(x0.reshape(-1, 3)[:, :, None], S, axis=2)).to(dev)
+            self._grad_call(lam_m1, x0_d, None, 0.5 * self.dt * self.dt, grad)
+        return grad
+
+
+class EnsembleLoss(torch.autograd.Function):
+    """``loss[S] = EnsembleLoss.apply(theta, problem)`` with ``theta = log E [E, S]``."""
+
+    @staticmethod
+    def forward(ctx, theta, problem):
+        problem.set_theta(theta)
+        loss = problem.forward()
+        ctx.problem = problem
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        grad = ctx.problem.backward(grad_out)
+        return grad.to(grad_out.device), None

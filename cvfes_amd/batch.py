@@ -1,0 +1,164 @@
+"""Batched ensemble of uneven meshes advanced by one persistent launch per step.
+
+``BatchedEnsemble`` takes a list of assembled :class:`MembraneSystem` (each with its own sample
+count, operator, time step, damping and load tables), numbers their wave tasks once
+(``cvfes_batch_plan``) and advances all of them together:
+
+* ``run(n, graph=True)`` replays HIP graphs that hold ``block`` batched steps (a multiple of 3, so
+  the rotating state roles come back to the start) and a device step counter increment; the
+  remainder runs as eager launches;
+* ``autotune()`` times candidate persistent grid sizes and keeps the fastest;
+* ``run_per_mesh(n)`` is the comparison path: each mesh's own single-mesh step kernel launched per
+  mesh per step.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import CvfesBatchMesh, check, ptr
+from .system import LoadTables, MembraneSystem
+
+__all__ = ["BatchedEnsemble"]
+
+
+class BatchedEnsemble:
+    def __init__(self, systems, loads, dts, methods="matfree", betas=0.0, damps=0.0, device=None):
+        self.systems = list(systems)
+        n = len(self.systems)
+        if n < 1:
+            raise ValueError("empty ensemble")
+        as_list = lambda v: list(v) if isinstance(v, (list, tuple, np.ndarray)) else [v] * n
+        self.loads = list(loads)
+        self.dts = [float(x) for x in as_list(dts)]
+        self.methods = as_list(methods)
+        self.betas = [float(x) for x in as_list(betas)]
+        self.damps = [float(x) for x in as_list(damps)]
+        self.device = torch.device(device) if device is not None else self.systems[0].device
+        self.lib = _lib.lib()
+        self.n_waves = 0
+        self.phase = 0
+        self.step_index = 0
+        self._graphs = {}
+        self.counter = torch.zeros(1, dtype=torch.int64, device=self.device)
+
+    @property
+    def stream(self):
+        return _lib.stream_handle(self.device)
+
+    def initial_state(self, u0s=None, du0s=None):
+        """Each mesh's ``u_0``, ``u_{-1}`` (zero start by default), then plan the batch."""
+        for i, s in enumerate(self.systems):
+            u0 = np.zeros(3 * s.n_nodes) if u0s is None else u0s[i]
+            du0 = np.zeros(3 * s.n_nodes) if du0s is None else du0s[i]
+            s.initial_state(u0, du0, self.loads[i], self.dts[i], beta=self.betas[i],
+                            damp=self.damps[i], method=self.methods[i])
+        self._plan()
+        self.phase = 0
+        self.step_index = 0
+
+    def _plan(self):
+        n = len(self.systems)
+        arr = (CvfesBatchMesh * n)()
+        for i, s in enumerate(self.systems):
+            m = arr[i]
+            m.sys = s._make_struct()
+            m.method = MembraneSystem.method_id(self.methods[i])
+            u, up, work = s.state
+            m.state[0], m.state[1], m.state[2] = u.data_ptr(), work.data_ptr(), up.data_ptr()
+            m.load = self.loads[i].struct(0)
+            m.dt, m.beta, m.damp = self.dts[i], self.betas[i], self.damps[i]
+        self._bufs = [(s.state[0], s.state[2], s.state[1]) for s in self.systems]   # state[0..2]
+        self.dev_meshes = torch.empty(ctypes.sizeof(arr), dtype=torch.uint8, device=self.device)
+        n_tasks = ctypes.c_int64(0)
+        check(self.lib.cvfes_batch_plan(self.stream, arr, n, self.dev_meshes.data_ptr(),
+                                        ctypes.byref(n_tasks)))
+        self.host_meshes = arr
+        self.n_tasks = int(n_tasks.value)
+        self._drop_graphs()
+
+    def _drop_graphs(self):
+        for g in self._graphs.values():
+            self.lib.cvfes_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self._drop_graphs()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------------------------------
+    def _eager(self, n):
+        for _ in range(n):
+            check(self.lib.cvfes_batch_step(self.stream, self.dev_meshes.data_ptr(),
+                                            len(self.systems), self.n_tasks, self.phase, None,
+                                            self.step_index, self.n_waves))
+            self.phase = (self.phase + 1) % 3
+            self.step_index += 1
+
+    def _graph(self, block):
+        key = (block, self.phase, self.n_waves)
+        if key not in self._graphs:
+            g = ctypes.c_void_p()
+            check(self.lib.cvfes_batch_graph_create(self.dev_meshes.data_ptr(), len(self.systems),
+                                                    self.n_tasks, self.counter.data_ptr(), block,
+                                                    self.phase, self.n_waves, ctypes.byref(g)))
+            self._graphs[key] = g.value
+        return self._graphs[key]
+
+    def run(self, n, graph=True, block=99):
+        """Advance every mesh by ``n`` steps."""
+        if graph and n >= block:
+            if block % 3:
+                raise ValueError("graph block must be a multiple of 3")
+            self.counter.fill_(self.step_index)
+            g = self._graph(block)
+            for _ in range(n // block):
+                check(self.lib.cvfes_graph_launch(g, self.stream))
+                self.step_index += block
+            n -= (n // block) * block
+        self._eager(n)
+        self._sync_roles()
+
+    def _sync_roles(self):
+        p = self.phase
+        for s, bufs in zip(self.systems, self._bufs):
+            s.state = [bufs[p], bufs[(p + 2) % 3], bufs[(p + 1) % 3]]
+
+    def run_per_mesh(self, n):
+        """Comparison path: every mesh's own single-mesh kernel, one launch per mesh per step."""
+        for k in range(n):
+            for i, s in enumerate(self.systems):
+                s.step(self.loads[i], self.dts[i], beta=self.betas[i], damp=self.damps[i],
+                       method=self.methods[i], offset=self.step_index + k, nsteps=1)
+        self.step_index += n
+        self._plan()          # descriptors follow each mesh's current buffer roles
+        self.phase = 0
+
+    # ------------------------------------------------------------------------------------------
+    def autotune(self, candidates=None, steps=6):
+        """Pick the persistent grid size (wavefronts) with the shortest batched step."""
+        if candidates is None:
+            candidates = [0, 256 * 8, 256 * 32, 256 * 64]
+        best = None
+        saved = (self.phase, self.step_index)
+        for w in candidates:
+            self.n_waves = w
+            self._eager(1)
+            torch.cuda.synchronize(self.device)
+            t0 = time.perf_counter()
+            self._eager(steps)
+            torch.cuda.synchronize(self.device)
+            dt = (time.perf_counter() - t0) / steps
+            if best is None or dt < best[1]:
+                best = (w, dt)
+        self.n_waves = best[0]
+        # the tuning steps advanced the state: restart the bookkeeping from the current roles
+        self._sync_roles()
+        self.tuned = {"n_waves": best[0], "ms_per_step": best[1] * 1e3, "from_step": saved[1]}
+        return self.tuned

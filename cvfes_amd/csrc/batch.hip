@@ -1,0 +1,234 @@
+// Batched ensemble: many meshes of uneven size, each with its own sample count, operator, time
+// step and loads, advance together in one persistent launch per step.
+//
+// Work is cut into wave tasks: (mesh, block row, chunk of 64*VW samples), VW = 2 for even sample
+// counts, else 1.  Tasks are numbered mesh by mesh, row by row (cvfes_batch_plan), so a contiguous
+// task range is a contiguous band of rows of one or two meshes.  Each wavefront of the persistent
+// grid takes one contiguous range; wave order is XCD-contiguous, so neighbouring rows (which share
+// gathered columns) run on one XCD and its L2.  Inside a task the row is wave-uniform: the mesh
+// descriptor, row pointers and column indices come in through scalar loads, and the row arithmetic
+// is the single-mesh step_row (rowops.h).
+//
+// Buffer roles rotate with `phase` (0, 1, 2): u = state[phase], u_prev = state[(phase+2)%3],
+// out = state[(phase+1)%3].  The step index used for the load tables is *counter + k_local (or
+// k_local without a counter).  A captured graph holds a multiple of 3 steps plus one counter
+// increment, so a replay leaves the roles and the counter ready for the next replay.
+#include "rowops.h"
+
+#include <algorithm>
+
+namespace cvfes {
+namespace {
+
+template <bool MF, bool BETA, bool DAMP, int VW>
+__device__ __forceinline__ void batch_task(const cvfes_batch_mesh_t &d, int phase, int64_t n,
+                                           int32_t row, int64_t s, bool active) {
+    StepArgs a;
+    a.k = RowArgs{d.sys.n_nodes, d.sys.n_smp, d.sys.indptr, d.sys.indices, d.sys.values,
+                  d.sys.cptr,    d.sys.celem, d.sys.khat,   d.sys.coef};
+    a.u = d.state[phase];
+    a.up = d.state[(phase + 2) % 3];
+    a.out = d.state[(phase + 1) % 3];
+    a.lm = d.sys.lm;
+    a.lm_sstride = d.sys.lm_sstride;
+    a.f1 = d.sys.f1;
+    a.fixed = d.sys.fixed;
+    a.load = d.load;
+    a.bdy_u = d.sys.bdy_u;
+    a.dt2 = d.dt * d.dt;
+    a.bdt = d.beta / d.dt;
+    a.ddt = d.damp / d.dt;
+    step_row<MF, BETA, DAMP, VW>(a, row, s, active, n);
+}
+
+template <int VW>
+__device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int phase, int64_t n,
+                                               int32_t row, int64_t s, bool active) {
+    const bool mf = d.method == CVFES_METHOD_MATFREE;
+    const bool beta = d.beta != 0.0, damp = d.damp != 0.0;
+    if (mf) {
+        if (beta) {
+            if (damp) batch_task<true, true, true, VW>(d, phase, n, row, s, active);
+            else batch_task<true, true, false, VW>(d, phase, n, row, s, active);
+        } else {
+            if (damp) batch_task<true, false, true, VW>(d, phase, n, row, s, active);
+            else batch_task<true, false, false, VW>(d, phase, n, row, s, active);
+        }
+    } else {
+        if (beta) {
+            if (damp) batch_task<false, true, true, VW>(d, phase, n, row, s, active);
+            else batch_task<false, true, false, VW>(d, phase, n, row, s, active);
+        } else {
+            if (damp) batch_task<false, false, true, VW>(d, phase, n, row, s, active);
+            else batch_task<false, false, false, VW>(d, phase, n, row, s, active);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes,
+                                                        int n_mesh, int64_t n_tasks, int phase,
+                                                        const int64_t *__restrict__ counter,
+                                                        int64_t k_local) {
+    constexpr int64_t kWavesPerBlock = kBlock / 64;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    const int64_t wave = __builtin_amdgcn_readfirstlane(
+        static_cast<int32_t>(xcd_block() * kWavesPerBlock + threadIdx.x / 64));
+    const int lane = threadIdx.x & 63;
+    const int64_t per = (n_tasks + waves - 1) / waves;
+    const int64_t t0 = wave * per;
+    const int64_t t1 = t0 + per < n_tasks ? t0 + per : n_tasks;
+    if (t0 >= t1) return;
+    const int64_t n = (counter ? counter[0] : 0) + k_local;
+    int lo = 0, hi = n_mesh - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (meshes[mid].task0 <= t0) lo = mid;
+        else hi = mid - 1;
+    }
+    int mi = lo;
+    for (int64_t t = t0; t < t1; ++t) {
+        while (mi + 1 < n_mesh && t >= meshes[mi + 1].task0) ++mi;
+        const cvfes_batch_mesh_t &d = meshes[mi];
+        const int64_t local = t - d.task0;
+        const int32_t row = static_cast<int32_t>(local / d.n_chunks);
+        const int64_t chunk = local - static_cast<int64_t>(row) * d.n_chunks;
+        const int64_t S = d.sys.n_smp;
+        if (d.vw == 2) {
+            int64_t s = chunk * 128 + 2 * lane;
+            const bool active = s < S;
+            if (!active) s = S - 2;
+            batch_dispatch<2>(d, phase, n, row, s, active);
+        } else {
+            int64_t s = chunk * 64 + lane;
+            const bool active = s < S;
+            if (!active) s = S - 1;
+            batch_dispatch<1>(d, phase, n, row, s, active);
+        }
+    }
+}
+
+__global__ void k_counter_add(int64_t *counter, int64_t by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += by;
+}
+
+unsigned waves_to_blocks(int64_t n_waves) {
+    int64_t b = (n_waves + 3) / 4;
+    b = (b + kNumXcd - 1) / kNumXcd * kNumXcd;
+    return static_cast<unsigned>(b < kNumXcd ? kNumXcd : b);
+}
+
+int default_waves() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    return cus * 16;
+}
+
+}  // namespace
+}  // namespace cvfes
+
+using namespace cvfes;
+
+extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh,
+                                void *dev_meshes, int64_t *n_tasks) {
+    CVFES_REQUIRE(meshes && n_mesh >= 1 && dev_meshes && n_tasks, "cvfes_batch_plan: bad arguments");
+    int64_t t = 0;
+    for (int i = 0; i < n_mesh; ++i) {
+        cvfes_batch_mesh_t &m = meshes[i];
+        const cvfes_system_t &s = m.sys;
+        CVFES_REQUIRE(s.n_nodes >= 1 && s.n_smp >= 1 && s.indptr && s.indices && s.lm && s.f1,
+                      "cvfes_batch_plan: mesh %d incomplete", i);
+        CVFES_REQUIRE(s.n_nodes * 3 < (int64_t(1) << 31), "cvfes_batch_plan: mesh %d too large", i);
+        CVFES_REQUIRE(m.method == CVFES_METHOD_STORED || m.method == CVFES_METHOD_MATFREE,
+                      "cvfes_batch_plan: mesh %d unknown method", i);
+        CVFES_REQUIRE(m.method == CVFES_METHOD_MATFREE || s.values, "cvfes_batch_plan: mesh %d has no values", i);
+        CVFES_REQUIRE(m.method == CVFES_METHOD_STORED || (s.cptr && s.celem && s.khat && s.coef),
+                      "cvfes_batch_plan: mesh %d lacks matrix-free data", i);
+        CVFES_REQUIRE(m.state[0] && m.state[1] && m.state[2], "cvfes_batch_plan: mesh %d state", i);
+        CVFES_REQUIRE(m.load.p && (!m.load.ga || m.load.qa) && (!m.load.gb || m.load.qb),
+                      "cvfes_batch_plan: mesh %d load tables", i);
+        CVFES_REQUIRE(m.dt > 0.0, "cvfes_batch_plan: mesh %d dt", i);
+        CVFES_REQUIRE(s.lm_sstride == 0 || s.lm_sstride == 1, "cvfes_batch_plan: mesh %d lm_sstride", i);
+        m.vw = (s.n_smp % 2 == 0) ? 2 : 1;
+        m.n_chunks = (s.n_smp + 64 * m.vw - 1) / (64 * m.vw);
+        m.task0 = t;
+        t += s.n_nodes * m.n_chunks;
+    }
+    *n_tasks = t;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemcpyAsync(dev_meshes, meshes, sizeof(cvfes_batch_mesh_t) * n_mesh,
+                                    hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return fail(CVFES_ERR_HIP, "cvfes_batch_plan: %s", hipGetErrorString(err));
+    return CVFES_OK;
+}
+
+extern "C" int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks,
+                                int phase, const int64_t *counter, int64_t k_local,
+                                int64_t n_waves) {
+    CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && n_tasks >= 0, "cvfes_batch_step: bad arguments");
+    CVFES_REQUIRE(phase >= 0 && phase < 3, "cvfes_batch_step: phase must be 0, 1 or 2");
+    if (n_tasks == 0) return CVFES_OK;
+    const int64_t w = n_waves > 0 ? n_waves : default_waves();
+    hipLaunchKernelGGL(k_batch_step, dim3(waves_to_blocks(w)), dim3(kBlock), 0,
+                       static_cast<hipStream_t>(stream),
+                       static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks, phase,
+                       counter, k_local);
+    return check_launch("k_batch_step");
+}
+
+extern "C" int cvfes_counter_add(void *stream, int64_t *counter, int64_t by) {
+    CVFES_REQUIRE(counter, "cvfes_counter_add: null counter");
+    hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       counter, by);
+    return check_launch("k_counter_add");
+}
+
+extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int64_t n_tasks,
+                                        int64_t *counter, int64_t block_steps, int phase0,
+                                        int64_t n_waves, void **graph) {
+    CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && counter && graph, "cvfes_batch_graph_create: bad arguments");
+    CVFES_REQUIRE(block_steps >= 3 && block_steps % 3 == 0,
+                  "cvfes_batch_graph_create: block_steps must be a positive multiple of 3");
+    CVFES_REQUIRE(phase0 >= 0 && phase0 < 3, "cvfes_batch_graph_create: phase0 must be 0, 1 or 2");
+    const int64_t w = n_waves > 0 ? n_waves : default_waves();
+    hipStream_t cs;
+    hipError_t err = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (err != hipSuccess) return fail(CVFES_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(err));
+    err = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) {
+        hipStreamDestroy(cs);
+        return fail(CVFES_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(err));
+    }
+    for (int64_t k = 0; k < block_steps; ++k)
+        hipLaunchKernelGGL(k_batch_step, dim3(waves_to_blocks(w)), dim3(kBlock), 0, cs,
+                           static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks,
+                           static_cast<int>((phase0 + k) % 3), counter, k);
+    hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, cs, counter, block_steps);
+    hipGraph_t g = nullptr;
+    err = hipStreamEndCapture(cs, &g);
+    hipGraphExec_t exec = nullptr;
+    if (err == hipSuccess) err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (g) hipGraphDestroy(g);
+    hipStreamDestroy(cs);
+    if (err != hipSuccess) return fail(CVFES_ERR_HIP, "graph capture: %s", hipGetErrorString(err));
+    *graph = exec;
+    return CVFES_OK;
+}
+
+extern "C" int cvfes_graph_launch(void *graph, void *stream) {
+    CVFES_REQUIRE(graph, "cvfes_graph_launch: null graph");
+    hipError_t err = hipGraphLaunch(static_cast<hipGraphExec_t>(graph), static_cast<hipStream_t>(stream));
+    if (err != hipSuccess) return fail(CVFES_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(err));
+    return CVFES_OK;
+}
+
+extern "C" int cvfes_graph_destroy(void *graph) {
+    if (!graph) return CVFES_OK;
+    hipError_t err = hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph));
+    if (err != hipSuccess) return fail(CVFES_ERR_HIP, "hipGraphExecDestroy: %s", hipGetErrorString(err));
+    return CVFES_OK;
+}

@@ -14,114 +14,30 @@
 //   METHOD_MATFREE: values rebuilt per block as sum_c coef[celem[c], s] * khat[c][ij]
 //                   (sample-independent khat, scalar loads; coef [E][S] is cached across the 3
 //                   rows that touch an element).
-#include "common.h"
+#include "rowops.h"
 
 namespace cvfes {
 namespace {
 
-struct RowArgs {
-    int64_t n_nodes;
-    int64_t n_smp;
-    const int32_t *indptr;
-    const int32_t *indices;
-    const double *values;
-    const int32_t *cptr;
-    const int32_t *celem;
-    const double *khat;
-    const double *coef;
-};
-
-struct StepArgs {
-    RowArgs k;
-    const double *u;      // u_n   [N][3][S]
-    const double *up;     // u_n-1 [N][3][S]
-    double *out;          // u_n+1 (step) or y = K u (spmv)
-    const double *lm;
-    int64_t lm_sstride;
-    const double *f1;
-    const uint8_t *fixed;
-    cvfes_load_t load;
-    double bdy_u;
-    double dt2;
-    double bdt;           // beta / dt
-    double ddt;           // damp / dt
-};
-
-template <bool MATFREE, int VW>
-__device__ __forceinline__ void block_values(const RowArgs &k, int32_t p, int64_t s,
-                                             double (&kv)[9][VW]) {
-    if constexpr (!MATFREE) {
-        const double *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
-#pragma unroll
-        for (int ij = 0; ij < 9; ++ij) {
-            const Vec<VW> t = load_vec<VW>(base + ij * k.n_smp);
-#pragma unroll
-            for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
-        }
-    } else {
-#pragma unroll
-        for (int ij = 0; ij < 9; ++ij)
-#pragma unroll
-            for (int v = 0; v < VW; ++v) kv[ij][v] = 0.0;
-        const int32_t c0 = k.cptr[p], c1 = k.cptr[p + 1];
-        for (int32_t c = c0; c < c1; ++c) {
-            const int64_t e = k.celem[c];
-            const Vec<VW> cf = load_vec<VW>(k.coef + e * k.n_smp + s);
-            const double *kh = k.khat + 9 * static_cast<int64_t>(c);
-#pragma unroll
-            for (int ij = 0; ij < 9; ++ij) {
-                const double h = kh[ij];
-#pragma unroll
-                for (int v = 0; v < VW; ++v) kv[ij][v] = fma(cf.v[v], h, kv[ij][v]);
-            }
-        }
-    }
-}
-
-// acc[i][v] = sum over the row's blocks of K_p (i, :) . x_col, with x = u + bdt (u - up) when BETA.
-template <bool MATFREE, bool BETA, int VW>
-__device__ __forceinline__ void row_product(const RowArgs &k, const double *__restrict__ u,
-                                            const double *__restrict__ up, double bdt, int32_t row,
-                                            int64_t s, double (&acc)[3][VW]) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) acc[i][v] = 0.0;
-    const int32_t p0 = k.indptr[row], p1 = k.indptr[row + 1];
-    for (int32_t p = p0; p < p1; ++p) {
-        const int64_t col = k.indices[p];
-        double x[3][VW];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int64_t off = (col * 3 + j) * k.n_smp + s;
-            const Vec<VW> xu = load_vec<VW>(u + off);
-            if constexpr (BETA) {
-                const Vec<VW> xp = load_vec<VW>(up + off);
-#pragma unroll
-                for (int v = 0; v < VW; ++v) x[j][v] = fma(bdt, xu.v[v] - xp.v[v], xu.v[v]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < VW; ++v) x[j][v] = xu.v[v];
-            }
-        }
-        double kv[9][VW];
-        block_values<MATFREE, VW>(k, p, s, kv);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int v = 0; v < VW; ++v) acc[i][v] = fma(kv[3 * i + j][v], x[j][v], acc[i][v]);
-    }
+// Lanes per block row.  Packed (UNIFORM = false): S/VW lanes, so a wave may span several rows.
+// Wave-uniform (UNIFORM = true): S/VW rounded up to whole wavefronts; lanes past the last sample
+// are inactive (they read the last sample group and do not store), which keeps every row on
+// whole wavefronts for any S.
+template <bool UNIFORM, int VW>
+__host__ __device__ __forceinline__ int64_t lanes_per_row(int64_t n_smp) {
+    const int64_t l = n_smp / VW;
+    return UNIFORM ? (l + 63) / 64 * 64 : l;
 }
 
 template <bool UNIFORM, int VW>
 __device__ __forceinline__ bool row_and_sample(int64_t n_nodes, int64_t n_smp, int32_t &row,
-                                               int64_t &s) {
-    const int64_t lpr = n_smp / VW;   // lanes per block row
+                                               int64_t &s, bool &active) {
+    const int64_t lpr = lanes_per_row<UNIFORM, VW>(n_smp);
     const int64_t g = xcd_block() * kBlock + threadIdx.x;
     int64_t r = g / lpr;
     s = (g - r * lpr) * VW;
+    active = s < n_smp;
+    if (!active) s = n_smp - VW;
     if constexpr (UNIFORM) r = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(r));
     row = static_cast<int32_t>(r);
     return r < n_nodes;
@@ -131,7 +47,8 @@ template <bool MATFREE, bool UNIFORM, int VW>
 __global__ __launch_bounds__(kBlock) void k_spmv(StepArgs a) {
     int32_t row;
     int64_t s;
-    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s)) return;
+    bool active;
+    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
     double acc[3][VW];
     row_product<MATFREE, false, VW>(a.k, a.u, nullptr, 0.0, row, s, acc);
 #pragma unroll
@@ -139,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(StepArgs a) {
         Vec<VW> o;
 #pragma unroll
         for (int v = 0; v < VW; ++v) o.v[v] = acc[i][v];
-        store_vec<VW>(a.out + (static_cast<int64_t>(row) * 3 + i) * a.k.n_smp + s, o);
+        if (active) store_vec<VW>(a.out + (static_cast<int64_t>(row) * 3 + i) * a.k.n_smp + s, o);
     }
 }
 
@@ -147,36 +64,9 @@ template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
 __global__ __launch_bounds__(kBlock) void k_step(StepArgs a, int64_t step) {
     int32_t row;
     int64_t s;
-    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s)) return;
-    double acc[3][VW];
-    row_product<MATFREE, BETA, VW>(a.k, a.u, a.up, a.bdt, row, s, acc);
-    const double pn = a.load.p[step];
-    const double qa = a.load.ga ? a.load.qa[step] : 0.0;
-    const double qb = a.load.gb ? a.load.qb[step] : 0.0;
-    const bool fixed = a.fixed != nullptr && a.fixed[row] != 0;
-    const int64_t nsm = a.k.n_smp;
-    const double *lmp = a.lm + (a.lm_sstride ? static_cast<int64_t>(row) * nsm + s : row);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const int64_t off = (static_cast<int64_t>(row) * 3 + i) * nsm + s;
-        const Vec<VW> uu = load_vec<VW>(a.u + off);
-        const Vec<VW> pp = load_vec<VW>(a.up + off);
-        const int64_t d = 3 * static_cast<int64_t>(row) + i;
-        double fi = pn * a.f1[d];
-        if (a.load.ga) fi += qa * a.load.ga[d];
-        if (a.load.gb) fi += qb * a.load.gb[d];
-        Vec<VW> o;
-#pragma unroll
-        for (int v = 0; v < VW; ++v) {
-            const double m = a.lm_sstride ? lmp[v] : lmp[0];
-            double force = fi;
-            if constexpr (DAMP) force -= a.ddt * (uu.v[v] - pp.v[v]);
-            const double rhs = a.dt2 * (force - acc[i][v]) + m * (2.0 * uu.v[v] - pp.v[v]);
-            double un = m != 0.0 ? rhs / m : 0.0;
-            o.v[v] = fixed ? a.bdy_u : un;
-        }
-        store_vec<VW>(a.out + off, o);
-    }
+    bool active;
+    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    step_row<MATFREE, BETA, DAMP, VW>(a, row, s, active, step);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -224,7 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
     typedef typename VecT<VW>::type V;
     int32_t row;
     int64_t s;
-    if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s)) return;
+    bool active;
+    if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
     const RowArgs &k = a.k;
     const int64_t S = k.n_smp;
     const int32_t p0 = k.indptr[row];
@@ -350,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
             un = m != 0.0 ? rhs / m : 0.0;
         }
         if (fixed) un = splat<VW>(a.bdy_u);
-        *reinterpret_cast<V *>(a.out + (static_cast<int64_t>(row) * 3 + i) * S + s) = un;
+        if (active) *reinterpret_cast<V *>(a.out + (static_cast<int64_t>(row) * 3 + i) * S + s) = un;
     }
 }
 
@@ -437,15 +328,22 @@ RowArgs row_args(const cvfes_system_t *sys) {
 }
 
 // Dispatch helpers ------------------------------------------------------------------------------
+// Wave-uniform rows unless padding to whole wavefronts would idle more than a quarter of the lanes.
+template <int VW>
+bool use_uniform(int64_t n_smp) {
+    const int64_t l = n_smp / VW, padded = (l + 63) / 64 * 64;
+    return 4 * l >= 3 * padded;
+}
+
 template <bool MF, bool UNI, int VW>
 void launch_spmv(hipStream_t st, const StepArgs &a) {
-    const int64_t work = a.k.n_nodes * (a.k.n_smp / VW);
+    const int64_t work = a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp);
     hipLaunchKernelGGL((k_spmv<MF, UNI, VW>), dim3(grid_for(work)), dim3(kBlock), 0, st, a);
 }
 
 template <bool MF, int VW>
 void launch_spmv_u(hipStream_t st, const StepArgs &a) {
-    if ((a.k.n_smp / VW) % 64 == 0)
+    if (use_uniform<VW>(a.k.n_smp))
         launch_spmv<MF, true, VW>(st, a);
     else
         launch_spmv<MF, false, VW>(st, a);
@@ -462,7 +360,7 @@ void spmv_dispatch(hipStream_t st, bool mf, const StepArgs &a) {
 
 template <bool MF, bool UNI, bool BETA, bool DAMP, int VW>
 void launch_step(hipStream_t st, const StepArgs &a, int64_t step) {
-    const int64_t work = a.k.n_nodes * (a.k.n_smp / VW);
+    const int64_t work = a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp);
     hipLaunchKernelGGL((k_step<MF, UNI, BETA, DAMP, VW>), dim3(grid_for(work)), dim3(kBlock), 0,
                        st, a, step);
 }
@@ -480,7 +378,7 @@ void step_bd(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool da
 
 template <bool MF, int VW>
 void step_u(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
-    if ((a.k.n_smp / VW) % 64 == 0)
+    if (use_uniform<VW>(a.k.n_smp))
         step_bd<MF, true, VW>(st, a, step, beta, damp);
     else
         step_bd<MF, false, VW>(st, a, step, beta, damp);
@@ -488,7 +386,7 @@ void step_u(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool dam
 
 template <bool MF, bool NT, int MAXB, int VW>
 void launch_rowload(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
-    const int64_t work = a.k.n_nodes * (a.k.n_smp / VW);
+    const int64_t work = a.k.n_nodes * lanes_per_row<true, VW>(a.k.n_smp);
     const dim3 g(grid_for(work)), b(kBlock);
     if (beta) {
         if (damp) hipLaunchKernelGGL((k_step_rowload<MF, true, true, NT, MAXB, VW>), g, b, 0, st, a, step);

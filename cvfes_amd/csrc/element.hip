@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_element_matrices(
 __global__ __launch_bounds__(kBlock) void k_element_coef(
     int64_t n_elms, int64_t n_smp, const int32_t *__restrict__ elems,
     const double *__restrict__ vthick, int64_t tss, const double *__restrict__ vE,
-    const double *__restrict__ elm_E, double *__restrict__ coef) {
+    const double *__restrict__ elm_E, int log_modulus, double *__restrict__ coef) {
     const int64_t g = xcd_block() * kBlock + threadIdx.x;
     if (g >= n_elms * n_smp) return;
     const int64_t e = g / n_smp;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_element_coef(
     const double w = 1.0 / 3.0;
     double out;
     if (elm_E) {
-        const double E = elm_E[g];
+        const double E = log_modulus ? exp(elm_E[g]) : elm_E[g];
         out = (hq0 * E) * w + (hq1 * E) * w + (hq2 * E) * w;
     } else {
         const double E0 = vE[n[0] * n_smp + s], E1 = vE[n[1] * n_smp + s], E2 = vE[n[2] * n_smp + s];
@@ -186,14 +186,15 @@ extern "C" int cvfes_element_matrices(void *stream, int64_t n_elms, const double
 
 extern "C" int cvfes_element_coef(void *stream, int64_t n_elms, int64_t n_smp, const int32_t *elems,
                                   const double *vthick, int64_t thick_sstride, const double *vE,
-                                  const double *elm_E, double *coef) {
+                                  const double *elm_E, int log_modulus, double *coef) {
     CVFES_REQUIRE(n_elms >= 0 && n_smp >= 1, "cvfes_element_coef: bad sizes");
     CVFES_REQUIRE(thick_sstride == 0 || thick_sstride == 1, "cvfes_element_coef: thick_sstride must be 0 or 1");
     if (n_elms == 0) return CVFES_OK;
     CVFES_REQUIRE(elems && vthick && coef && (vE || elm_E), "cvfes_element_coef: null pointer");
+    CVFES_REQUIRE(!log_modulus || elm_E, "cvfes_element_coef: log_modulus needs elm_E");
     hipLaunchKernelGGL(k_element_coef, dim3(grid_for(n_elms * n_smp)), dim3(kBlock), 0,
                        static_cast<hipStream_t>(stream), n_elms, n_smp, elems, vthick,
-                       thick_sstride, vE, elm_E, coef);
+                       thick_sstride, vE, elm_E, log_modulus, coef);
     return check_launch("k_element_coef");
 }
 

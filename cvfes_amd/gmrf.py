@@ -97,9 +97,41 @@ class GMRFSample:
         self.E, self.z, self.tau, self.var = E, z, tau, var
 
 
+def _cg_torch(A, rhs, device, tol=1e-13, maxiter=500):
+    """Batched conjugate gradients with the sparse operator on ``device`` (torch CSR)."""
+    import torch
+    A = A.tocsr()
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr, dtype=torch.int64),
+                                 torch.as_tensor(A.indices, dtype=torch.int64),
+                                 torch.as_tensor(A.data), size=A.shape, device=device)
+    b = torch.as_tensor(rhs).to(device)
+    x = torch.zeros_like(b)
+    r = b.clone()
+    p = r.clone()
+    rr = (r * r).sum(0)
+    r0 = torch.sqrt(torch.clamp(rr, min=1e-300))
+    for _ in range(maxiter):
+        if bool((torch.sqrt(rr) <= tol * r0).all()):
+            break
+        ap = At @ p
+        pap = (p * ap).sum(0)
+        alpha = torch.where(pap > 0, rr / torch.where(pap > 0, pap, torch.ones_like(pap)), torch.zeros_like(pap))
+        x += alpha * p
+        r -= alpha * ap
+        rr_new = (r * r).sum(0)
+        beta = torch.where(rr > 0, rr_new / torch.where(rr > 0, rr, torch.ones_like(rr)), torch.zeros_like(rr))
+        p = r + beta * p
+        rr = rr_new
+    return x.cpu().numpy()
+
+
 def sample_element_E(elements: np.ndarray, n_smp: int, seed: int, E0: float = 7.0e6,
-                     cov: float = 0.10, tau: float | None = None) -> GMRFSample:
-    """Draw ``n_smp`` per-element Young's modulus samples (layout ``[E, S]``, sample-fastest)."""
+                     cov: float = 0.10, tau: float | None = None, device=None) -> GMRFSample:
+    """Draw ``n_smp`` per-element Young's modulus samples (layout ``[E, S]``, sample-fastest).
+
+    The random numbers always come from the seeded host generator; ``device`` (e.g. ``"cuda"``)
+    only moves the conjugate-gradient solve there for large ensembles.
+    """
     A, B = dual_laplacian_system(elements)
     n_elm = A.shape[0]
     var = math.log1p(cov * cov)
@@ -108,7 +140,8 @@ def sample_element_E(elements: np.ndarray, n_smp: int, seed: int, E0: float = 7.
     rng = np.random.default_rng(seed)
     xi1 = rng.standard_normal((n_elm, n_smp))
     xi2 = rng.standard_normal((B.shape[0], n_smp))
-    y = _cg(A, xi1 + B.T @ xi2)
+    rhs = xi1 + B.T @ xi2
+    y = _cg(A, rhs) if device is None else _cg_torch(A, rhs, device)
     z = y / math.sqrt(tau)
     E = E0 * np.exp(z - 0.5 * var)
     return GMRFSample(np.ascontiguousarray(E), np.ascontiguousarray(z), float(tau), var)

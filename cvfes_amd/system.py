@@ -147,7 +147,7 @@ class MembraneSystem:
                     raise ValueError("one of coef, vE, elmE is required")
                 self.coef = torch.empty((self.n_elms, S), dtype=f64, device=dev)
                 check(self.lib.cvfes_element_coef(st, self.n_elms, S, ptr(self.elems), ptr(self.vthick),
-                                                  self.thick_sstride, ptr(vE_d), ptr(elmE_d),
+                                                  self.thick_sstride, ptr(vE_d), ptr(elmE_d), 0,
                                                   ptr(self.coef)))
                 torch.cuda.current_stream(dev).synchronize()
             # --- lumped mass and unit-pressure load -----------------------------------------------
@@ -167,6 +167,18 @@ class MembraneSystem:
         self.state = None
 
     # ------------------------------------------------------------------------------------------
+    def set_log_modulus(self, theta):
+        """Re-form ``coef`` from per-element ``theta = log E`` (device tensor ``[E,S]``) and, for the
+        stored operator, the block values."""
+        theta = theta.detach().to(self.device, torch.float64).contiguous()
+        if theta.shape != (self.n_elms, self.n_smp):
+            raise ValueError("theta must be [n_elms, n_smp]")
+        check(self.lib.cvfes_element_coef(self.stream, self.n_elms, self.n_smp, ptr(self.elems),
+                                          ptr(self.vthick), self.thick_sstride, None, ptr(theta), 1,
+                                          ptr(self.coef)))
+        if self.values is not None:
+            self.assemble_values()
+
     def assemble_values(self):
         """Form (or re-form) the stored block values ``[nnzb][9][S]`` from ``coef``."""
         S = self.n_smp

@@ -108,11 +108,11 @@ int cvfes_element_matrices(void *stream, int64_t n_elms, const double *nodes, co
  * interpolated at the edge-midpoint Gauss points, or E[e,s] * sum_q w_q h_q when elm_E != NULL.
  * Replaces: physicsSolver.py:154-165,200-202 (elmGE, elmGThick, coefK).
  *   vthick  [N][S] (thick_sstride = 1) or [N] (0);  vE [N][S] (ignored when elm_E given);
- *   elm_E   [E][S] or NULL.
+ *   elm_E   [E][S] or NULL; with log_modulus = 1 it holds log E (the parameter of the gradient).
  */
 int cvfes_element_coef(void *stream, int64_t n_elms, int64_t n_smp, const int32_t *elems,
                        const double *vthick, int64_t thick_sstride, const double *vE,
-                       const double *elm_E, double *coef);
+                       const double *elm_E, int log_modulus, double *coef);
 
 /*
  * Lumped mass and unit-pressure nodal load, pulled per node over incident elements.
@@ -169,6 +169,68 @@ int cvfes_initial_state(void *stream, const cvfes_system_t *sys, int method, con
  */
 int cvfes_step(void *stream, const cvfes_system_t *sys, int method, double **state,
                const cvfes_load_t *load, int64_t nsteps, double dt, double beta, double damp);
+
+/*
+ * One step of the adjoint recurrence of cvfes_step (no counterpart in the reference, which has no
+ * gradient; it is the transpose of the step above, K being symmetric):
+ *   lambda = (2-gamma) mu1 - (1-gamma) mu2 - K A (mu1 + (beta/dt)(mu1 - mu2)) + w (obs_u - obs_ref)
+ *   mu_out = lambda with fixed dofs zeroed when project = 1
+ * A = dt^2/lm (column scale), gamma = dt damp / lm.  obs_u/obs_ref/obs_w (weights [S]) add the
+ * observation term of L = sum w/2 |u_n - u_obs|^2 and may be NULL.  Arrays are [N][3][S].
+ */
+int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int method, const double *mu1,
+                       const double *mu2, double *mu_out, const double *obs_u,
+                       const double *obs_ref, const double *obs_w, double dt, double beta,
+                       double damp, int project);
+
+/*
+ * Gradient with respect to theta = log E per element and sample (no counterpart in the reference):
+ *   grad[e,s] -= coef[e,s] * sum_ab (nu_scale mu/lm)_a^T khat_e(a,b) x_b,
+ *   x = x_cur + (beta/dt)(x_cur - x_prev)  (x_prev may be NULL: x = x_cur).
+ * elems [E][3], cpos [9E] (block slot of element e's (a,b) in khat), grad [E][S] accumulated.
+ */
+int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const int32_t *elems,
+                     const int32_t *cpos, const double *mu, const double *x_cur,
+                     const double *x_prev, double dt, double beta, double nu_scale, double *grad);
+
+/*
+ * Batched ensemble (no counterpart in the reference, which advances one mesh per process): many
+ * meshes of uneven size, each with its own samples, operator, time step and loads, advance together
+ * in one persistent launch per step.  Host description of one mesh:
+ */
+typedef struct cvfes_batch_mesh {
+    cvfes_system_t sys;    /* device arrays of the mesh                                      */
+    int64_t method;        /* CVFES_METHOD_*                                                  */
+    double *state[3];      /* state buffers; with phase p: u = state[p], u_prev = state[p+2 mod 3],
+                              output = state[p+1 mod 3]                                       */
+    cvfes_load_t load;     /* tables indexed by the absolute step                             */
+    double dt, beta, damp;
+    int64_t task0;         /* set by cvfes_batch_plan: first wave task of the mesh             */
+    int64_t n_chunks;      /* set by cvfes_batch_plan: wave tasks per block row                */
+    int64_t vw;            /* set by cvfes_batch_plan: samples per lane (2 for even counts)    */
+} cvfes_batch_mesh_t;
+
+/* Number the wave tasks (mesh, row, 64*vw-sample chunk) of `meshes` (host array, updated in place)
+ * and copy the array to `dev_meshes` (device, n_mesh * sizeof(cvfes_batch_mesh_t) bytes).  Blocks
+ * until the copy is done. */
+int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void *dev_meshes,
+                     int64_t *n_tasks);
+
+/* One step of every mesh (one launch).  Step index for the load tables: counter[0] + k_local
+ * (k_local alone when counter is NULL).  n_waves: persistent grid size in wavefronts (0 = 16 per
+ * compute unit). */
+int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks, int phase,
+                     const int64_t *counter, int64_t k_local, int64_t n_waves);
+
+/* counter[0] += by on the stream (a kernel, so that it can sit inside a captured graph). */
+int cvfes_counter_add(void *stream, int64_t *counter, int64_t by);
+
+/* Capture block_steps (a multiple of 3) batched steps starting at phase0, followed by
+ * counter += block_steps, into an executable HIP graph returned in *graph. */
+int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int64_t n_tasks, int64_t *counter,
+                             int64_t block_steps, int phase0, int64_t n_waves, void **graph);
+int cvfes_graph_launch(void *graph, void *stream);
+int cvfes_graph_destroy(void *graph);
 
 #ifdef __cplusplus
 }

@@ -160,3 +160,18 @@ def test_kernel_variants_match_oracle(case128, method, variant):
 def test_structure_limits(case128):
     s = case128.sys
     assert (s.max_row_blocks, s.max_offdiag_contrib, s.max_diag_contrib) == (7, 2, 6)
+
+
+@pytest.mark.parametrize("n_smp", [97, 100])
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_padded_wavefront_rows_match_oracle(require_gpu, method, n_smp):
+    """Sample counts that are not a multiple of the wavefront (rows padded to whole wavefronts)."""
+    c = Case("tube-64x32-s16", n_smp=n_smp, seed=3)
+    out, snaps = _run_both(c, 150, method, beta=1e-6, damp=20.0, dt=3e-6, record=(1, 60))
+    for k, u in out.items():
+        assert rel_err(u, snaps[k]) < REL_U, f"step {k}"
+    import torch
+    x = np.random.default_rng(1).standard_normal((c.N, 3, c.S))
+    y = c.sys.spmv(torch.as_tensor(x).cuda(), method).cpu().numpy()
+    yref = so.multiply_by_vector(c.indptr, c.indices, c.K, x.reshape(-1, c.S))
+    assert rel_err(y.reshape(-1, c.S), yref) < 1e-13
