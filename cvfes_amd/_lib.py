@@ -22,9 +22,10 @@ METHODS = {"stored": METHOD_STORED, "bcsr": METHOD_STORED, "matfree": METHOD_MAT
 
 VARIANT_AUTO, VARIANT_GENERIC, VARIANT_ROWLOAD, VARIANT_ROWLOAD_NT = 0, 1, 2, 3
 VARIANT_ROWLOAD1, VARIANT_ROWLOAD1_NT = 4, 5
+VARIANT_LDS, VARIANT_LDS2 = 6, 7
 VARIANTS = {"auto": VARIANT_AUTO, "generic": VARIANT_GENERIC, "rowload": VARIANT_ROWLOAD,
             "rowload_nt": VARIANT_ROWLOAD_NT, "rowload1": VARIANT_ROWLOAD1,
-            "rowload1_nt": VARIANT_ROWLOAD1_NT}
+            "rowload1_nt": VARIANT_ROWLOAD1_NT, "lds": VARIANT_LDS, "lds2": VARIANT_LDS2}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -45,7 +46,7 @@ class CvfesSystem(ctypes.Structure):
         ("cptr", _vp), ("celem", _vp), ("khat", _vp), ("coef", _vp),
         ("lm", _vp), ("lm_sstride", _i64), ("f1", _vp), ("fixed", _vp), ("bdy_u", _f64),
         ("max_row_blocks", _i64), ("max_offdiag_contrib", _i64), ("max_diag_contrib", _i64),
-        ("variant", _i64),
+        ("variant", _i64), ("nptr", _vp), ("nent", _vp), ("cslot", _vp), ("max_node_elems", _i64),
     ]
 
 

@@ -82,32 +82,6 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a, int64_t step) {
 //   blocks per row <= MAXB, contributions per off-diagonal block <= 2,
 //   contributions per diagonal block <= MAXB - 1 (a node has at least as many neighbours as
 //   incident triangles in a manifold triangulation).
-typedef double dbl2 __attribute__((ext_vector_type(2)));
-
-template <int VW> struct VecT;
-template <> struct VecT<1> { typedef double type; };
-template <> struct VecT<2> { typedef dbl2 type; };
-
-template <int VW, bool NT>
-__device__ __forceinline__ typename VecT<VW>::type ldv(const double *p) {
-    typedef typename VecT<VW>::type V;
-    if constexpr (NT)
-        return __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
-    else
-        return *reinterpret_cast<const V *>(p);
-}
-
-template <int VW>
-__device__ __forceinline__ typename VecT<VW>::type splat(double x) {
-    if constexpr (VW == 2) return dbl2{x, x};
-    else return x;
-}
-
-template <int VW>
-__device__ __forceinline__ typename VecT<VW>::type sel(bool keep, typename VecT<VW>::type v) {
-    return keep ? v : splat<VW>(0.0);
-}
-
 template <bool MATFREE, bool BETA, bool DAMP, bool NT, int MAXB, int VW>
 __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t step) {
     constexpr int MAXD = MAXB - 1;
@@ -218,31 +192,19 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
             }
         }
     }
-    const double pn = a.load.p[step];
-    const double qa = a.load.ga ? a.load.qa[step] : 0.0;
-    const double qb = a.load.gb ? a.load.qb[step] : 0.0;
-    const bool fixed = a.fixed != nullptr && a.fixed[row] != 0;
-    const V m = a.lm_sstride ? ldv<VW, false>(a.lm + static_cast<int64_t>(row) * S + s)
-                             : splat<VW>(a.lm[row]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const int64_t d = 3 * static_cast<int64_t>(row) + i;
-        double fi = pn * a.f1[d];
-        if (a.load.ga) fi += qa * a.load.ga[d];
-        if (a.load.gb) fi += qb * a.load.gb[d];
-        V force = splat<VW>(fi);
-        if constexpr (DAMP) force -= a.ddt * (uo[i] - po[i]);
-        const V rhs = a.dt2 * (force - acc[i]) + m * (2.0 * uo[i] - po[i]);
-        V un;
-        if constexpr (VW == 2) {
-            un.x = m.x != 0.0 ? rhs.x / m.x : 0.0;
-            un.y = m.y != 0.0 ? rhs.y / m.y : 0.0;
-        } else {
-            un = m != 0.0 ? rhs / m : 0.0;
-        }
-        if (fixed) un = splat<VW>(a.bdy_u);
-        if (active) *reinterpret_cast<V *>(a.out + (static_cast<int64_t>(row) * 3 + i) * S + s) = un;
-    }
+    finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
+}
+
+// Matrix-free step with each row's element data staged in LDS (step_row_lds); one wavefront per
+// (row, 64*VW samples), so rows must be wave-uniform.
+template <bool BETA, bool DAMP, int VW, int MAXB, int MAXE>
+__global__ __launch_bounds__(kBlock) void k_step_lds(StepArgs a, LdsArgs l, int64_t step) {
+    __shared__ LdsRow<VW, MAXE> s_row[kBlock / 64];
+    int32_t row;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    step_row_lds<BETA, DAMP, VW, MAXB, MAXE>(a, l, s_row[threadIdx.x / 64], row, s, active, step);
 }
 
 // values[p][ij][s] = sum_c coef[celem[c]][s] * khat[c][ij]; one thread per (block, sample).
@@ -403,6 +365,47 @@ void rowload_maxb(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bo
     else launch_rowload<MF, NT, 10, VW>(st, a, step, beta, damp);
 }
 
+template <bool BETA, bool DAMP, int VW, int MAXB, int MAXE>
+void launch_lds_inst(hipStream_t st, const StepArgs &a, const LdsArgs &l, int64_t step) {
+    const int64_t work = a.k.n_nodes * lanes_per_row<true, VW>(a.k.n_smp);
+    hipLaunchKernelGGL((k_step_lds<BETA, DAMP, VW, MAXB, MAXE>), dim3(grid_for(work)), dim3(kBlock),
+                       0, st, a, l, step);
+}
+
+template <int VW, int MAXB, int MAXE>
+void launch_lds_bd(hipStream_t st, const StepArgs &a, const LdsArgs &l, int64_t step, bool beta,
+                   bool damp) {
+    if (beta) {
+        if (damp) launch_lds_inst<true, true, VW, MAXB, MAXE>(st, a, l, step);
+        else launch_lds_inst<true, false, VW, MAXB, MAXE>(st, a, l, step);
+    } else {
+        if (damp) launch_lds_inst<false, true, VW, MAXB, MAXE>(st, a, l, step);
+        else launch_lds_inst<false, false, VW, MAXB, MAXE>(st, a, l, step);
+    }
+}
+
+// LDS slots needed by the structure: 1 = (MAXB 7, MAXE 6), 2 = (MAXB 10, MAXE 9), 0 = too large.
+int lds_slots(const cvfes_system_t *sys) {
+    if (!sys->nptr || !sys->nent || !sys->cslot || sys->max_node_elems < 1 || sys->max_row_blocks < 1)
+        return 0;
+    if (sys->max_row_blocks <= 7 && sys->max_node_elems <= 6) return 1;
+    if (sys->max_row_blocks <= 10 && sys->max_node_elems <= 9) return 2;
+    return 0;
+}
+
+void launch_lds(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step,
+                bool beta, bool damp, bool two) {
+    const LdsArgs l{sys->nptr, sys->nent, sys->cslot};
+    const int slots = lds_slots(sys);
+    if (two) {
+        if (slots == 1) launch_lds_bd<2, 7, 6>(st, a, l, step, beta, damp);
+        else launch_lds_bd<2, 10, 9>(st, a, l, step, beta, damp);
+    } else {
+        if (slots == 1) launch_lds_bd<1, 7, 6>(st, a, l, step, beta, damp);
+        else launch_lds_bd<1, 10, 9>(st, a, l, step, beta, damp);
+    }
+}
+
 // Row-load kernels apply when rows are wave-uniform (S % 128 == 0) and the structure fits their
 // slots; returns the MAXB instance to use, or 0.
 int rowload_slots(const cvfes_system_t *sys) {
@@ -422,6 +425,14 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
     // row-load kernel and non-temporal value loads; the matrix-free operator with the generic
     // kernel (the row-load form needs too many registers for enough waves in flight).
     if (v == CVFES_VARIANT_AUTO) v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
+    if (mf && (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2) && lds_slots(sys)) {
+        const bool two = v == CVFES_VARIANT_LDS2 && a.k.n_smp % 2 == 0;
+        launch_lds(st, sys, a, step, beta, damp, two);
+        return;
+    }
+    // LDS staging only exists for the matrix-free operator (and structures within its limits)
+    if (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2)
+        v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
     if (maxb && v != CVFES_VARIANT_GENERIC) {
         const bool one = v == CVFES_VARIANT_ROWLOAD1 || v == CVFES_VARIANT_ROWLOAD1_NT;
         const bool nt = v == CVFES_VARIANT_ROWLOAD_NT || v == CVFES_VARIANT_ROWLOAD1_NT;
@@ -518,7 +529,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     CVFES_REQUIRE(nsteps >= 0, "cvfes_step: nsteps < 0");
     CVFES_REQUIRE(dt > 0.0, "cvfes_step: dt must be positive");
     CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_step: lm_sstride must be 0 or 1");
-    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_ROWLOAD1_NT,
+    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_LDS2,
                   "cvfes_step: unknown variant %lld", static_cast<long long>(sys->variant));
     hipStream_t st = static_cast<hipStream_t>(stream);
     StepArgs a{};

@@ -139,4 +139,197 @@ __device__ __forceinline__ void step_row(const StepArgs &a, int32_t row, int64_t
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Ext-vector forms (one value or two consecutive samples per lane) used by the row-load and
+// LDS-staged kernels.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <int VW> struct VecT;
+template <> struct VecT<1> { typedef double type; };
+template <> struct VecT<2> { typedef dbl2 type; };
+
+template <int VW, bool NT>
+__device__ __forceinline__ typename VecT<VW>::type ldv(const double *p) {
+    typedef typename VecT<VW>::type V;
+    if constexpr (NT)
+        return __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+    else
+        return *reinterpret_cast<const V *>(p);
+}
+
+template <int VW>
+__device__ __forceinline__ typename VecT<VW>::type splat(double x) {
+    if constexpr (VW == 2) return dbl2{x, x};
+    else return x;
+}
+
+template <int VW>
+__device__ __forceinline__ typename VecT<VW>::type sel(bool keep, typename VecT<VW>::type v) {
+    return keep ? v : splat<VW>(0.0);
+}
+
+// Central-difference update of one block row from its product acc = K (u + bdt (u - up)), with
+// the row's own u/up already loaded; stores u_{n+1} when `active`.
+template <bool DAMP, int VW>
+__device__ __forceinline__ void finish_row(const StepArgs &a, int32_t row, int64_t s, bool active,
+                                           int64_t step, const typename VecT<VW>::type (&uo)[3],
+                                           const typename VecT<VW>::type (&po)[3],
+                                           const typename VecT<VW>::type (&acc)[3]) {
+    typedef typename VecT<VW>::type V;
+    const int64_t S = a.k.n_smp;
+    const double pn = a.load.p[step];
+    const double qa = a.load.ga ? a.load.qa[step] : 0.0;
+    const double qb = a.load.gb ? a.load.qb[step] : 0.0;
+    const bool fixed = a.fixed != nullptr && a.fixed[row] != 0;
+    const V m = a.lm_sstride ? ldv<VW, false>(a.lm + static_cast<int64_t>(row) * S + s)
+                             : splat<VW>(a.lm[row]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int64_t d = 3 * static_cast<int64_t>(row) + i;
+        double fi = pn * a.f1[d];
+        if (a.load.ga) fi += qa * a.load.ga[d];
+        if (a.load.gb) fi += qb * a.load.gb[d];
+        V force = splat<VW>(fi);
+        if constexpr (DAMP) force -= a.ddt * (uo[i] - po[i]);
+        const V rhs = a.dt2 * (force - acc[i]) + m * (2.0 * uo[i] - po[i]);
+        V un;
+        if constexpr (VW == 2) {
+            un.x = m.x != 0.0 ? rhs.x / m.x : 0.0;
+            un.y = m.y != 0.0 ? rhs.y / m.y : 0.0;
+        } else {
+            un = m != 0.0 ? rhs / m : 0.0;
+        }
+        if (fixed) un = splat<VW>(a.bdy_u);
+        if (active) *reinterpret_cast<V *>(a.out + (static_cast<int64_t>(row) * 3 + i) * S + s) = un;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Matrix-free row step with the row's element data staged in LDS.
+//
+// The generic matrix-free row walks blocks and, inside each block, element contributions, and
+// every contribution is a dependent chain (element id -> coefficient load -> FMAs), about 25 memory
+// round trips per row.  Here one wavefront owns one row (wave-uniform, 64*VW samples) and:
+//   1. loads, all at once, the coefficient of every element incident to the row node (one
+//      coalesced load per element), the unit element blocks and element slots of the row's
+//      contributions (contiguous: one lane per value), and the neighbour states;
+//   2. parks coefficients, element blocks and slots in its LDS slice;
+//   3. rebuilds each block as sum_c coef[slot_c] * khat_c from LDS (element blocks and slots are
+//      read at wave-uniform addresses, i.e. broadcast) and accumulates the product with the
+//      neighbour state.
+// A row node with ne incident elements has exactly 3*ne contributions (each element adds one to
+// the blocks (row, b) of its three nodes b).  Limits (exact maxima, checked on the host): blocks
+// per row <= MAXB, incident elements per node <= MAXE.
+struct LdsArgs {
+    const int32_t *nptr;    // [N+1] node -> incident (element, local) list
+    const int32_t *nent;    // [3E]  e*3 + local
+    const int32_t *cslot;   // [9E]  slot of contribution c's element in its row node's list
+};
+
+template <int VW, int MAXE>
+struct LdsRow {
+    typename VecT<VW>::type coef[MAXE][64];
+    double khat[3 * MAXE * 9];
+    int32_t slot[3 * MAXE];
+};
+
+// Orders this wavefront's LDS writes before its later reads (and earlier reads before later
+// writes when a wavefront reuses its slice); only the own wavefront shares the slice.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool BETA, bool DAMP, int VW, int MAXB, int MAXE>
+__device__ __forceinline__ void step_row_lds(const StepArgs &a, const LdsArgs &l, LdsRow<VW, MAXE> &sl,
+                                             int32_t row, int64_t s, bool active, int64_t step) {
+    constexpr int MAXC = 3 * MAXE;
+    constexpr int KH = (MAXC * 9 + 63) / 64;     // element-block values per lane
+    typedef typename VecT<VW>::type V;
+    const int lane = threadIdx.x & 63;
+    const RowArgs &k = a.k;
+    const int64_t S = k.n_smp;
+    const int32_t p0 = k.indptr[row];
+    const int32_t nb = k.indptr[row + 1] - p0;
+    const int32_t q0 = l.nptr[row];
+    const int32_t ne = l.nptr[row + 1] - q0;
+    V uo[3], po[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int64_t off = (static_cast<int64_t>(row) * 3 + i) * S + s;
+        uo[i] = ldv<VW, false>(a.u + off);
+        po[i] = ldv<VW, false>(a.up + off);
+    }
+    V acc[3] = {splat<VW>(0.0), splat<VW>(0.0), splat<VW>(0.0)};
+    if (nb > 0 && ne > 0) {
+        const int32_t c0 = k.cptr[p0];
+        const int32_t nc = k.cptr[p0 + nb] - c0;
+        // 1. every load of the row, issued together
+        V cf[MAXE];
+#pragma unroll
+        for (int e = 0; e < MAXE; ++e) {
+            const int64_t el = l.nent[q0 + (e < ne ? e : ne - 1)] / 3;
+            cf[e] = ldv<VW, false>(k.coef + el * S + s);
+        }
+        double kh[KH];
+#pragma unroll
+        for (int j = 0; j < KH; ++j) {
+            const int idx = lane + 64 * j;
+            kh[j] = idx < 9 * nc ? k.khat[9 * static_cast<int64_t>(c0) + idx] : 0.0;
+        }
+        const int32_t slot = lane < nc ? l.cslot[c0 + lane] : 0;
+        V w[MAXB][3];
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const int64_t col = k.indices[p0 + (b < nb ? b : nb - 1)];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int64_t off = (col * 3 + j) * S + s;
+                const V xu = ldv<VW, false>(a.u + off);
+                if constexpr (BETA) {
+                    const V xp = ldv<VW, false>(a.up + off);
+                    w[b][j] = xu + a.bdt * (xu - xp);
+                } else {
+                    w[b][j] = xu;
+                }
+            }
+        }
+        // 2. stage coefficients, element blocks and slots in this wavefront's LDS slice
+        wave_lds_sync();
+#pragma unroll
+        for (int e = 0; e < MAXE; ++e) sl.coef[e][lane] = cf[e];
+#pragma unroll
+        for (int j = 0; j < KH; ++j) {
+            const int idx = lane + 64 * j;
+            if (idx < MAXC * 9) sl.khat[idx] = kh[j];
+        }
+        if (lane < MAXC) sl.slot[lane] = slot;
+        wave_lds_sync();
+        // 3. rebuild each block from LDS and accumulate its product
+        int32_t cb = 0;
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            if (b < nb) {
+                const int32_t ce = k.cptr[p0 + b + 1] - c0;
+                V kv[9];
+#pragma unroll
+                for (int ij = 0; ij < 9; ++ij) kv[ij] = splat<VW>(0.0);
+                for (int32_t j = cb; j < ce; ++j) {
+                    const V c = sl.coef[sl.slot[j]][lane];
+                    const double *kb = &sl.khat[9 * j];
+#pragma unroll
+                    for (int ij = 0; ij < 9; ++ij) kv[ij] += c * kb[ij];
+                }
+                cb = ce;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) acc[i] += kv[3 * i + j] * w[b][j];
+            }
+        }
+    }
+    finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
+}
+
 }  // namespace cvfes

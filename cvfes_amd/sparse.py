@@ -86,5 +86,16 @@ class BlockMaps:
         np.cumsum(np.bincount(flat, minlength=n_nodes), out=self.nptr[1:])
         self.nent = norder.astype(np.int32)                    # e*3 + local
         self.n_contrib = 9 * n_elm
+        # slot of each contribution's element in its row node's incident-element list
+        node_of_q = np.repeat(np.arange(n_nodes, dtype=np.int64), np.diff(self.nptr))
+        elem_of_q = norder // 3
+        slot_of_q = np.arange(3 * n_elm, dtype=np.int64) - self.nptr[node_of_q]
+        qkeys = node_of_q * n_elm + elem_of_q
+        qorder = np.argsort(qkeys)
+        row_of_block = np.repeat(np.arange(n_nodes, dtype=np.int64), np.diff(indptr))
+        ckeys = row_of_block[pos[order]] * n_elm + (order // 9)
+        where = np.searchsorted(qkeys[qorder], ckeys)
+        self.cslot = slot_of_q[qorder][where].astype(np.int32)
+        self.max_node_elems = int(np.diff(self.nptr).max()) if n_nodes else 0
         self.cptr = self.cptr.astype(np.int32)
         self.nptr = self.nptr.astype(np.int32)

@@ -116,6 +116,8 @@ class MembraneSystem:
         self.cpos = _dev(maps.cpos, i32, dev)
         self.nptr = _dev(maps.nptr, i32, dev)
         self.nent = _dev(maps.nent, i32, dev)
+        self.cslot = _dev(maps.cslot, i32, dev)
+        self.max_node_elems = maps.max_node_elems
         fixed = np.zeros(self.n_nodes, np.uint8)
         fixed[np.asarray(boundary, np.int64)] = 1
         self.fixed = _dev(fixed, torch.uint8, dev)
@@ -202,7 +204,8 @@ class MembraneSystem:
             lm=ptr(self.lm), lm_sstride=self.thick_sstride, f1=ptr(self.f1), fixed=ptr(self.fixed),
             bdy_u=self.bdy_u, max_row_blocks=self.max_row_blocks,
             max_offdiag_contrib=self.max_offdiag_contrib, max_diag_contrib=self.max_diag_contrib,
-            variant=self.variant)
+            variant=self.variant, nptr=ptr(self.nptr), nent=ptr(self.nent), cslot=ptr(self.cslot),
+            max_node_elems=self.max_node_elems)
 
     def set_variant(self, variant):
         """Select the step kernel variant (``auto``, ``generic``, ``rowload``, ``rowload_nt``)."""

@@ -61,6 +61,12 @@ typedef struct cvfes_system {
     int64_t max_offdiag_contrib;
     int64_t max_diag_contrib;
     int64_t variant;          /* CVFES_VARIANT_*                                               */
+    /* Node -> incident element lists and, per contribution, the slot of its element in its row
+     * node's list (LDS-staged matrix-free kernel; may be NULL otherwise). */
+    const int32_t *nptr;      /* [N+1]                                                          */
+    const int32_t *nent;      /* [3E] e*3 + local index                                         */
+    const int32_t *cslot;     /* [9E]                                                           */
+    int64_t max_node_elems;   /* exact maximum of nptr[a+1] - nptr[a] (0 = unknown)             */
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */
@@ -70,6 +76,8 @@ typedef struct cvfes_system {
 #define CVFES_VARIANT_ROWLOAD_NT 3 /* as ROWLOAD, stored values streamed with non-temporal loads   */
 #define CVFES_VARIANT_ROWLOAD1 4   /* as ROWLOAD with one sample per lane (two waves per row)       */
 #define CVFES_VARIANT_ROWLOAD1_NT 5 /* as ROWLOAD1 with non-temporal value loads                   */
+#define CVFES_VARIANT_LDS 6        /* matrix-free: row coefficients and element blocks staged in LDS */
+#define CVFES_VARIANT_LDS2 7       /* as LDS with two samples per lane                               */
 
 /* Time-dependent nodal load: f_n = p[n] * f1 + qa[n] * ga + qb[n] * gb  (ga/gb may be NULL).
  * ga, gb are sample-independent [N][3] traction fields (e.g. two consecutive fluid wall-stress

@@ -144,7 +144,8 @@ def case128(require_gpu):
     return Case("tube-64x32-s16", n_smp=128, seed=9)
 
 
-@pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt"])
+@pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt",
+                                     "lds", "lds2"])
 @pytest.mark.parametrize("method", ["stored", "matfree"])
 def test_kernel_variants_match_oracle(case128, method, variant):
     """Every step-kernel variant at 128 samples (wave-uniform rows) against the oracle."""
