@@ -60,8 +60,8 @@ class CvfesBatchMesh(ctypes.Structure):
     """Mirror of ``cvfes_batch_mesh_t``."""
 
     _fields_ = [("sys", CvfesSystem), ("method", _i64), ("state", _vp * 3), ("load", CvfesLoad),
-                ("dt", _f64), ("beta", _f64), ("damp", _f64), ("task0", _i64), ("n_chunks", _i64),
-                ("vw", _i64)]
+                ("dt", _f64), ("beta", _f64), ("damp", _f64), ("task0", _i64), ("lanes", _i64),
+                ("vw", _i64), ("packed", _i64)]
 
 
 _SIGNATURES = {

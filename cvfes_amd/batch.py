@@ -7,9 +7,10 @@ count, operator, time step, damping and load tables), numbers their wave tasks o
 * ``run(n, graph=True)`` replays HIP graphs that hold ``block`` batched steps (a multiple of 3, so
   the rotating state roles come back to the start) and a device step counter increment; the
   remainder runs as eager launches;
-* ``autotune()`` times candidate persistent grid sizes and keeps the fastest;
-* ``run_per_mesh(n)`` is the comparison path: each mesh's own single-mesh step kernel launched per
-  mesh per step.
+* ``autotune()`` times candidate persistent grid sizes (from a few wavefronts per SIMD up to one
+  wavefront per task) and keeps the fastest;
+* ``run_per_mesh(n, method)`` is the comparison path: each mesh's own single-mesh step kernel
+  launched per mesh per step (with its own operator, or the one given).
 """
 from __future__ import annotations
 
@@ -130,12 +131,13 @@ class BatchedEnsemble:
         for s, bufs in zip(self.systems, self._bufs):
             s.state = [bufs[p], bufs[(p + 2) % 3], bufs[(p + 1) % 3]]
 
-    def run_per_mesh(self, n):
-        """Comparison path: every mesh's own single-mesh kernel, one launch per mesh per step."""
+    def run_per_mesh(self, n, method=None):
+        """Comparison path: every mesh's own single-mesh kernel, one launch per mesh per step
+        (``method`` overrides the meshes' operators, e.g. ``"stored"`` for the plain step)."""
         for k in range(n):
             for i, s in enumerate(self.systems):
                 s.step(self.loads[i], self.dts[i], beta=self.betas[i], damp=self.damps[i],
-                       method=self.methods[i], offset=self.step_index + k, nsteps=1)
+                       method=method or self.methods[i], offset=self.step_index + k, nsteps=1)
         self.step_index += n
         self._plan()          # descriptors follow each mesh's current buffer roles
         self.phase = 0
@@ -144,7 +146,8 @@ class BatchedEnsemble:
     def autotune(self, candidates=None, steps=6):
         """Pick the persistent grid size (wavefronts) with the shortest batched step."""
         if candidates is None:
-            candidates = [0, 256 * 8, 256 * 32, 256 * 64]
+            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+            candidates = [cus * w for w in (8, 16, 32, 64, 128)] + [self.n_tasks]
         best = None
         saved = (self.phase, self.step_index)
         for w in candidates:

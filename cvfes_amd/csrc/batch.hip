@@ -1,13 +1,15 @@
 // Batched ensemble: many meshes of uneven size, each with its own sample count, operator, time
 // step and loads, advance together in one persistent launch per step.
 //
-// Work is cut into wave tasks: (mesh, block row, chunk of 64*VW samples), VW = 2 for even sample
-// counts, else 1.  Tasks are numbered mesh by mesh, row by row (cvfes_batch_plan), so a contiguous
-// task range is a contiguous band of rows of one or two meshes.  Each wavefront of the persistent
-// grid takes one contiguous range; wave order is XCD-contiguous, so neighbouring rows (which share
-// gathered columns) run on one XCD and its L2.  Inside a task the row is wave-uniform: the mesh
-// descriptor, row pointers and column indices come in through scalar loads, and the row arithmetic
-// is the single-mesh step_row (rowops.h).
+// Work is cut into wave tasks of 64 lanes of one mesh; a lane owns VW consecutive samples of one
+// block row (VW = 2 for even sample counts, else 1).  A mesh whose samples fill at least 3/4 of
+// whole wavefronts gets wave-uniform rows (each row padded to whole wavefronts: the mesh
+// descriptor, row pointers and column indices come in through scalar loads); otherwise its rows
+// are packed back to back so that no lane idles (row data through vector loads), as the
+// single-mesh kernels do.  Tasks are numbered mesh by mesh, so a contiguous task range is a band
+// of rows of one or two meshes.  Each wavefront of the persistent grid takes one contiguous range;
+// wave order is XCD-contiguous, so neighbouring rows (which share gathered columns) run on one XCD
+// and its L2.  The row arithmetic is the single-mesh step_row (rowops.h).
 //
 // Buffer roles rotate with `phase` (0, 1, 2): u = state[phase], u_prev = state[(phase+2)%3],
 // out = state[(phase+1)%3].  The step index used for the load tables is *counter + k_local (or
@@ -79,30 +81,39 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const cvfes_batch_mesh_t 
     const int64_t t1 = t0 + per < n_tasks ? t0 + per : n_tasks;
     if (t0 >= t1) return;
     const int64_t n = (counter ? counter[0] : 0) + k_local;
-    int lo = 0, hi = n_mesh - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (meshes[mid].task0 <= t0) lo = mid;
-        else hi = mid - 1;
+    // Mesh of the first task: meshes are numbered in task order, so it is the number of meshes with
+    // task0 <= t0, minus one.  Each lane reads one descriptor's task0 (all loads of the wave issue
+    // together, 64 meshes per round) and a ballot counts them.
+    int mi = -1;
+    for (int base = 0; base < n_mesh; base += 64) {
+        const int m = base + lane;
+        const bool le = m < n_mesh && meshes[m].task0 <= t0;
+        mi += __popcll(__ballot(le));
     }
-    int mi = lo;
     for (int64_t t = t0; t < t1; ++t) {
         while (mi + 1 < n_mesh && t >= meshes[mi + 1].task0) ++mi;
         const cvfes_batch_mesh_t &d = meshes[mi];
-        const int64_t local = t - d.task0;
-        const int32_t row = static_cast<int32_t>(local / d.n_chunks);
-        const int64_t chunk = local - static_cast<int64_t>(row) * d.n_chunks;
+        // lane index inside the mesh; 32-bit (cvfes_batch_plan checks n_nodes * lanes < 2^31)
+        const uint32_t g = static_cast<uint32_t>((t - d.task0) * 64) + lane;
+        const uint32_t lanes = static_cast<uint32_t>(d.lanes);
         const int64_t S = d.sys.n_smp;
-        if (d.vw == 2) {
-            int64_t s = chunk * 128 + 2 * lane;
-            const bool active = s < S;
-            if (!active) s = S - 2;
-            batch_dispatch<2>(d, phase, n, row, s, active);
+        const int vw = static_cast<int>(d.vw);
+        if (d.packed) {
+            const uint32_t r = g / lanes;
+            int64_t s = static_cast<int64_t>(g - r * lanes) * vw;
+            const bool in = r < static_cast<uint32_t>(d.sys.n_nodes);
+            const bool active = in && s < S;
+            const int32_t row = in ? static_cast<int32_t>(r) : static_cast<int32_t>(d.sys.n_nodes - 1);
+            if (!active) s = S - vw;
+            if (vw == 2) batch_dispatch<2>(d, phase, n, row, s, active);
+            else batch_dispatch<1>(d, phase, n, row, s, active);
         } else {
-            int64_t s = chunk * 64 + lane;
+            const int32_t row = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(g / lanes));
+            int64_t s = static_cast<int64_t>(g - static_cast<uint32_t>(row) * lanes) * vw;
             const bool active = s < S;
-            if (!active) s = S - 1;
-            batch_dispatch<1>(d, phase, n, row, s, active);
+            if (!active) s = S - vw;
+            if (vw == 2) batch_dispatch<2>(d, phase, n, row, s, active);
+            else batch_dispatch<1>(d, phase, n, row, s, active);
         }
     }
 }
@@ -153,9 +164,13 @@ extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_
         CVFES_REQUIRE(m.dt > 0.0, "cvfes_batch_plan: mesh %d dt", i);
         CVFES_REQUIRE(s.lm_sstride == 0 || s.lm_sstride == 1, "cvfes_batch_plan: mesh %d lm_sstride", i);
         m.vw = (s.n_smp % 2 == 0) ? 2 : 1;
-        m.n_chunks = (s.n_smp + 64 * m.vw - 1) / (64 * m.vw);
+        const int64_t l = s.n_smp / m.vw, padded = (l + 63) / 64 * 64;
+        m.packed = 4 * l >= 3 * padded ? 0 : 1;
+        m.lanes = m.packed ? l : padded;
+        CVFES_REQUIRE(s.n_nodes * m.lanes < (int64_t(1) << 31) - 64,
+                      "cvfes_batch_plan: mesh %d has too many row lanes", i);
         m.task0 = t;
-        t += s.n_nodes * m.n_chunks;
+        t += (s.n_nodes * m.lanes + 63) / 64;
     }
     *n_tasks = t;
     hipStream_t st = static_cast<hipStream_t>(stream);

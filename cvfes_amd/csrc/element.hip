@@ -11,52 +11,10 @@
 //                   [0, 0, gk (b_a b_b + c_a c_b)]] / (4 A (1 - nu^2)),  g = (1-nu)/2, gk = k g
 //           global block T^T k_ab T with T rows (e1, e2, e3)  (physicsSolverSDampGPUs.cl:340-357)
 // which equals A * TT^T B^T D B TT of pyx:299-329 per unit stiffness coefficient.
-#include "common.h"
+#include "geometry.h"
 
 namespace cvfes {
 namespace {
-
-struct Frame {
-    double t[3][3];   // rows e1, e2, e3
-    double lx[3], ly[3];
-    double area;
-};
-
-__device__ __forceinline__ Frame element_frame(const double *__restrict__ nodes, int n0, int n1, int n2) {
-    Frame f;
-    double X[3][3];
-    const int ids[3] = {n0, n1, n2};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) X[a][k] = nodes[3 * static_cast<int64_t>(ids[a]) + k];
-    double e0[3], e1[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        e0[k] = X[2][k] - X[1][k];
-        e1[k] = X[0][k] - X[2][k];
-    }
-    double nrm = sqrt(e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) f.t[0][k] = e0[k] / nrm;
-    const double proj = e1[0] * f.t[0][0] + e1[1] * f.t[0][1] + e1[2] * f.t[0][2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) f.t[1][k] = e1[k] - proj * f.t[0][k];
-    nrm = sqrt(f.t[1][0] * f.t[1][0] + f.t[1][1] * f.t[1][1] + f.t[1][2] * f.t[1][2]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) f.t[1][k] = f.t[1][k] / nrm;
-    f.t[2][0] = f.t[0][1] * f.t[1][2] - f.t[0][2] * f.t[1][1];
-    f.t[2][1] = f.t[0][2] * f.t[1][0] - f.t[0][0] * f.t[1][2];
-    f.t[2][2] = f.t[0][0] * f.t[1][1] - f.t[0][1] * f.t[1][0];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        f.lx[a] = X[a][0] * f.t[0][0] + X[a][1] * f.t[0][1] + X[a][2] * f.t[0][2];
-        f.ly[a] = X[a][0] * f.t[1][0] + X[a][1] * f.t[1][1] + X[a][2] * f.t[1][2];
-    }
-    f.area = ((f.lx[1] * f.ly[2] - f.lx[2] * f.ly[1]) - (f.lx[0] * f.ly[2] - f.lx[2] * f.ly[0]) +
-              (f.lx[0] * f.ly[1] - f.lx[1] * f.ly[0])) * 0.5;
-    return f;
-}
 
 __global__ __launch_bounds__(kBlock) void k_element_matrices(
     int64_t n_elms, const double *__restrict__ nodes, const int32_t *__restrict__ elems, double nu,

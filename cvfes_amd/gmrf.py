@@ -79,14 +79,16 @@ def _cg(A, rhs, tol=1e-13, maxiter=500):
     return x
 
 
-def mean_inverse_diagonal(A, seed: int = 12345, probes: int = 64, exact_limit: int = 4000) -> float:
-    """Mean of ``diag(A^-1)``: exact for ``n <= exact_limit``, else Hutchinson with Rademacher probes."""
+def mean_inverse_diagonal(A, seed: int = 12345, probes: int = 64, exact_limit: int = 4000,
+                          device=None) -> float:
+    """Mean of ``diag(A^-1)``: exact for ``n <= exact_limit``, else Hutchinson with Rademacher probes
+    (the probe solves run on ``device`` when one is given)."""
     n = A.shape[0]
     if n <= exact_limit:
         return float(np.mean(np.diag(np.linalg.inv(A.toarray()))))
     rng = np.random.default_rng(seed)
     v = rng.integers(0, 2, size=(n, probes)).astype(np.float64) * 2.0 - 1.0
-    y = _cg(A, v)
+    y = _cg(A, v) if device is None else _cg_torch(A, v, device)
     return float(np.mean(np.einsum("ij,ij->j", v, y)) / n)
 
 
@@ -136,7 +138,7 @@ def sample_element_E(elements: np.ndarray, n_smp: int, seed: int, E0: float = 7.
     n_elm = A.shape[0]
     var = math.log1p(cov * cov)
     if tau is None:
-        tau = mean_inverse_diagonal(A) / var
+        tau = mean_inverse_diagonal(A, device=device) / var
     rng = np.random.default_rng(seed)
     xi1 = rng.standard_normal((n_elm, n_smp))
     xi2 = rng.standard_normal((B.shape[0], n_smp))

@@ -214,13 +214,18 @@ typedef struct cvfes_batch_mesh {
     cvfes_load_t load;     /* tables indexed by the absolute step                             */
     double dt, beta, damp;
     int64_t task0;         /* set by cvfes_batch_plan: first wave task of the mesh             */
-    int64_t n_chunks;      /* set by cvfes_batch_plan: wave tasks per block row                */
+    int64_t lanes;         /* set by cvfes_batch_plan: lanes per block row                     */
     int64_t vw;            /* set by cvfes_batch_plan: samples per lane (2 for even counts)    */
+    int64_t packed;        /* set by cvfes_batch_plan: 1 = rows packed back to back in lanes,
+                              0 = each row padded to whole wavefronts                         */
 } cvfes_batch_mesh_t;
 
-/* Number the wave tasks (mesh, row, 64*vw-sample chunk) of `meshes` (host array, updated in place)
- * and copy the array to `dev_meshes` (device, n_mesh * sizeof(cvfes_batch_mesh_t) bytes).  Blocks
- * until the copy is done. */
+/* Number the wave tasks of `meshes` (host array, updated in place) and copy the array to
+ * `dev_meshes` (device, n_mesh * sizeof(cvfes_batch_mesh_t) bytes).  A wave task is 64 lanes of
+ * one mesh, each lane vw consecutive samples of one block row.  A mesh whose sample count fills at
+ * least 3/4 of whole wavefronts gets wave-uniform rows (lanes = S/vw rounded up to 64: row data
+ * through scalar loads); otherwise its rows are packed back to back (lanes = S/vw).  Blocks until
+ * the copy is done. */
 int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void *dev_meshes,
                      int64_t *n_tasks);
 

@@ -26,6 +26,14 @@ Pinning (what checks each function against something other than its own formula,
   (free flight under constant load, undamped and beta-damped single-degree-of-freedom systems).
 * ``gauss_properties`` / ``coef_k``: uniform-field closed form (``coefK = h*E``) and linear-field
   interpolation at edge midpoints.
+* ``assemble_update``: sample ``s`` equals ``assemble_reference`` (pinned above) on the mesh moved
+  by ``u_s``; rigid translation leaves it unchanged, a rigid rotation R maps every block to
+  R K R^T; planted mistakes (geometry of the wrong sample) are caught.
+* ``run_updated``: reduces to ``run`` when the interval exceeds the step count; the update
+  schedule (first re-formation at step ``interval``) is checked by counting re-formations.
+* ``calculate_stress``: zero for rigid translations; a homogeneous in-plane strain on a flat patch
+  gives ``E D eps`` in every element; the nodal force ``h A B^T sigma`` (rotated to global) equals
+  the element stiffness product ``K_e u_e`` of the pinned assembly.
 """
 from __future__ import annotations
 
@@ -35,6 +43,7 @@ __all__ = [
     "sparse_info_loops", "gauss_points", "gauss_properties", "d_matrix", "local_mass", "coef_k",
     "coef_k_element_E", "coordinate_transformation", "assemble_reference", "element_blocks_closed_form",
     "multiply_by_vector", "multiply_by_1d_vector", "initial_state", "step", "run",
+    "assemble_update", "run_updated", "calculate_stress",
 ]
 
 
@@ -347,3 +356,139 @@ def run(indptr, indices, K, LM, f1, pressures, u0, du0, dt, beta=0.0, damp=0.0, 
         if n + 1 in record:
             snaps[n + 1] = u.copy()
     return u, up, snaps
+
+
+# ----------------------------------------------------------------------------------------------
+# Updated geometry (optimizedSolidAssemble.pyx:606-819; physicsSolver.py:294-324)
+# ----------------------------------------------------------------------------------------------
+
+
+def _frames_batched(X):
+    """Frames ``T [..., 3, 3]`` of triangles ``X [..., 3 nodes, 3]`` (``pyx:606-651``)."""
+    e0 = X[..., 2, :] - X[..., 1, :]
+    e1 = X[..., 0, :] - X[..., 2, :]
+    t0 = e0 / np.sqrt((e0 * e0).sum(-1))[..., None]
+    t1 = e1 - (e1 * t0).sum(-1)[..., None] * t0
+    t1 = t1 / np.sqrt((t1 * t1).sum(-1))[..., None]
+    t2 = np.stack([t0[..., 1] * t1[..., 2] - t0[..., 2] * t1[..., 1],
+                   t0[..., 2] * t1[..., 0] - t0[..., 0] * t1[..., 2],
+                   t0[..., 0] * t1[..., 1] - t0[..., 1] * t1[..., 0]], axis=-1)
+    return np.stack([t0, t1, t2], axis=-2)
+
+
+def _local_batched(X, T):
+    """Area and B ``[..., 5, 9]`` of triangles ``X`` in frames ``T`` (``pyx:754-779``)."""
+    lx = np.einsum("...ak,...k->...a", X, T[..., 0, :])
+    ly = np.einsum("...ak,...k->...a", X, T[..., 1, :])
+    area = ((lx[..., 1] * ly[..., 2] - lx[..., 2] * ly[..., 1])
+            - (lx[..., 0] * ly[..., 2] - lx[..., 2] * ly[..., 0])
+            + (lx[..., 0] * ly[..., 1] - lx[..., 1] * ly[..., 0])) * 0.5
+    c1 = 2.0 * area
+    y = [ly[..., 1] - ly[..., 2], ly[..., 2] - ly[..., 0], ly[..., 0] - ly[..., 1]]
+    x = [lx[..., 2] - lx[..., 1], lx[..., 0] - lx[..., 2], lx[..., 1] - lx[..., 0]]
+    B = np.zeros(X.shape[:-2] + (5, 9))
+    for a in range(3):
+        B[..., 0, 3 * a] = y[a] / c1
+        B[..., 2, 3 * a + 1] = y[a] / c1
+        B[..., 3, 3 * a + 2] = y[a] / c1
+        B[..., 1, 3 * a + 1] = x[a] / c1
+        B[..., 2, 3 * a] = x[a] / c1
+        B[..., 4, 3 * a + 2] = x[a] / c1
+    return area, B
+
+
+def _big_T_batched(T):
+    TT = np.zeros(T.shape[:-2] + (9, 9))
+    for a in range(3):
+        TT[..., 3 * a:3 * a + 3, 3 * a:3 * a + 3] = T
+    return TT
+
+
+def assemble_update(nodes_s, elements, trac, D, coefK, indptr, indices, chunk=2048):
+    """``OptimizedSolidAssembleUpdate`` (``optimizedSolidAssemble.pyx:710-819``, with
+    ``assemblingUpdate`` ``:657-704``): per-sample stiffness ``K [nnzb, S, 3, 3]`` and traction load
+    ``f [ndof, S]`` on the per-sample geometry ``nodes_s [S, N, 3]`` (the mass is not re-formed)."""
+    nodes_s = np.asarray(nodes_s, float)
+    elements = np.asarray(elements, np.int64)
+    n_smp, n_nodes = nodes_s.shape[0], nodes_s.shape[1]
+    nnzb = indices.size
+    K = np.zeros((nnzb, n_smp, 3, 3))
+    f = np.zeros((3 * n_nodes, n_smp))
+    row_keys = np.repeat(np.arange(n_nodes), np.diff(indptr)) * n_nodes + indices
+    for e0 in range(0, elements.shape[0], chunk):
+        el = elements[e0:e0 + chunk]
+        X = np.moveaxis(nodes_s[:, el], 0, 1)                # [e, S, 3 nodes, 3]
+        T = _frames_batched(X)                               # [e, S, 3, 3]
+        TT = _big_T_batched(T)
+        area, B = _local_batched(X, T)
+        BTDB = np.einsum("eski,kl,eslj->esij", B, D, B)
+        lK = BTDB * (coefK[e0:e0 + chunk] * area)[..., None, None]
+        gK = np.einsum("eski,eskl,eslj->esij", TT, lK, TT)
+        lf = np.zeros(area.shape + (9,))
+        lf[..., 2] = lf[..., 5] = lf[..., 8] = trac / 3.0 * area
+        gf = np.einsum("esji,esj->esi", TT, lf)
+        for a in range(3):
+            for i in range(3):
+                np.add.at(f, 3 * el[:, a] + i, gf[:, :, 3 * a + i])
+            for b in range(3):
+                pos = np.searchsorted(row_keys, el[:, a] * n_nodes + el[:, b])
+                np.add.at(K, pos, gK[:, :, 3 * a:3 * a + 3, 3 * b:3 * b + 3])
+    return K, f
+
+
+def run_updated(nodes, elements, D, coefK, indptr, indices, K0, LM, f1, pressures, u0, du0, dt,
+                interval, beta=0.0, damp=0.0, bdy_dofs=None, bdy_u=0.0, record=None):
+    """Explicit steps with the stiffness and the unit-traction load re-formed on the moved geometry
+    ``X + u_s`` every ``interval`` steps (``physicsSolver.py:294-324``): a step counter starts at 0
+    after the initial assembly; before each step, if the counter has reached ``interval`` the
+    system is re-formed from the current ``u`` and the counter set to 1, otherwise it is
+    incremented (so the first re-formation happens before step ``interval``, 0-based).  The load of
+    step ``n`` is ``pressures[n] * f1`` with ``f1`` the unit-traction load of the current geometry
+    (EXTENSION: the reference CPU path applies its fixed ``appTraction``).  Returns
+    ``(u_N, u_{N-1}, snapshots, n_updates)``."""
+    nodes = np.asarray(nodes, float)
+    n_smp = K0.shape[1]
+    u, up, LHS = initial_state(indptr, indices, K0, LM, pressures[0] * f1, u0, du0, dt, beta, damp)
+    K, fs = K0, np.asarray(f1, float)[:, None]
+    counter, n_up = 0, 0
+    snaps = {}
+    record = set(record or ())
+    for n, p in enumerate(np.asarray(pressures, float)):
+        if counter >= interval:
+            moved = nodes[None] + u.T.reshape(n_smp, -1, 3)
+            K, fs = assemble_update(moved, elements, 1.0, D, coefK, indptr, indices)
+            counter, n_up = 1, n_up + 1
+        else:
+            counter += 1
+        unew = step(indptr, indices, K, LM, LHS, p * fs, u, up, dt, beta, damp, bdy_dofs, bdy_u)
+        up, u = u, unew
+        if n + 1 in record:
+            snaps[n + 1] = u.copy()
+    return u, up, snaps, n_up
+
+
+def calculate_stress(nodes_s, elements, D, ave_E, u):
+    """``OptimizedCalculateStress`` (``optimizedSolidAssemble.pyx:1116-1223``): global stress
+    components ``(xx, yy, xy, xz, yz)`` ``[E, S, 5]`` of the displacement ``u [ndof, S]`` on the
+    per-sample geometry ``nodes_s [S, N, 3]`` (``mesh.updateNodes``) with the element-average modulus
+    ``ave_E [E, S]`` (``physicsSolver.py:416-424``)."""
+    nodes_s = np.asarray(nodes_s, float)
+    elements = np.asarray(elements, np.int64)
+    n_smp = u.shape[1]
+    X = np.moveaxis(nodes_s[:, elements], 0, 1)              # [E, S, 3 nodes, 3]
+    T = _frames_batched(X)
+    TT = _big_T_batched(T)
+    area, B = _local_batched(X, T)
+    dofs = (3 * elements[:, :, None] + np.arange(3)).reshape(-1, 9)   # [E, 9]
+    gU = np.moveaxis(u[dofs], 2, 1)                           # [E, S, 9]
+    lU = np.einsum("esij,esj->esi", TT, gU)                   # dot1(TT, gU)
+    DB = np.einsum("ik,eskj->esij", D, B)
+    ls = np.einsum("esij,esj->esi", DB, lU) * ave_E[..., None]
+    St = np.zeros(ls.shape[:2] + (3, 3))
+    St[..., 0, 0] = ls[..., 0]
+    St[..., 1, 1] = ls[..., 1]
+    St[..., 0, 1] = St[..., 1, 0] = ls[..., 2]
+    St[..., 0, 2] = St[..., 2, 0] = ls[..., 3]
+    St[..., 1, 2] = St[..., 2, 1] = ls[..., 4]
+    G = np.einsum("eski,eskl,eslj->esij", T, St, T)           # T^T S T
+    return np.stack([G[..., 0, 0], G[..., 1, 1], G[..., 0, 1], G[..., 0, 2], G[..., 1, 2]], axis=-1)

@@ -118,3 +118,15 @@ def test_batched_mesh_matches_oracle(require_gpu):
                      bdy_dofs=bd)
     ug = systems[k].u.cpu().numpy().reshape(-1, S)
     assert np.max(np.abs(ug - u)) / np.max(np.abs(u)) < 1e-10
+
+
+def test_gmrf_device_solves_match_host(require_gpu):
+    """GMRF samples with the probe and sample solves on the GPU equal the host solves (CG
+    tolerance), on a mesh large enough for the stochastic mean-diagonal estimate."""
+    from cvfes_amd.gmrf import sample_element_E
+    nodes, el, bdy = tube_geometry(64, 40, 0.3, 3.0)
+    assert el.shape[0] > 4000
+    host = sample_element_E(el, 8, seed=5)
+    dev = sample_element_E(el, 8, seed=5, device="cuda")
+    assert abs(dev.tau - host.tau) <= 1e-10 * host.tau
+    assert np.max(np.abs(dev.z - host.z)) <= 1e-9 * np.max(np.abs(host.z))
