@@ -10,7 +10,16 @@ Output: one JSON line with sample-steps/s (``value``), the roofline of the step 
 HBM bytes per launch / average launch time from HIP events on the launch stream), the matrix-free
 variant measured the same way, and the CPU baseline (the C/OpenMP restatement of the reference CPU
 step timed on a bounded sample on this host) plus a full-size parity check of the GPU state against
-it after the baseline's steps.
+it after the baseline's steps.  ``roofline.traffic`` is the HBM traffic per launch measured with
+rocprofv3 counters (``profiles/pmc_traffic.json``, scaled by the same counters on a plain copy).
+
+Other workloads (``--workload``):
+  batch    the uneven 64-tube ensemble advanced by one persistent launch per step (HIP-graph blocks
+           of 99 steps, autotuned grid), against per-mesh launches of the plain stored step and of
+           the matrix-free step;
+  adjoint  forward + adjoint gradient of the 49,152-node tube (32 samples, 2,000 steps, 20
+           observations, checkpoints every 100 steps) and the adjoint / forward time ratio;
+  copy     a plain device copy of ``--copy-bytes`` (the calibration kernel for the counters).
 """
 from __future__ import annotations
 
@@ -39,6 +48,17 @@ def algorithmic_bytes(sysm, method):
     else:
         op = E * S * 8 + 9 * E * 9 * 8 + 4 * 9 * E + 4 * (nnzb + 1)   # coef, khat, celem, cptr
     return state + lm + small + op
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        return table[workload][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def time_steps(sysm, load, dt, beta, method, warmup, steps):
@@ -102,8 +122,10 @@ def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "adjoint", "copy"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--case", default="tube-256x400-s128")
     ap.add_argument("--method", default="stored", choices=["stored", "matfree"])
     ap.add_argument("--cpu-steps", type=int, default=4, help="CPU baseline steps (first is warm-up)")
@@ -117,6 +139,15 @@ def main():
               "exchange, so an N-GPU run is N independent sample shards; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "adjoint":
+        return run_adjoint(args)
+    if args.workload == "copy":
+        return run_copy(args)
+    args.steps = 200 if args.steps is None else args.steps
+    args.warmup = 20 if args.warmup is None else args.warmup
 
     import torch
     from cvfes_amd.configs import get_case
@@ -155,7 +186,8 @@ def main():
                    "nnzb": sysm.nnzb, "n_samples": S, "dt": dt, "beta": beta,
                    "operator": args.method, "layout": "values [nnzb][9][S], state [N][3][S]"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(case.name, args.method),
                      "algorithmic_bytes_per_launch": abytes, "kernel_ms": kern_ms,
                      "kernel": "k_step (fused block-CSR product + central-difference update)"},
     }
@@ -188,6 +220,145 @@ def main():
                                 "rel_max_err": float(np.max(np.abs(ug - u_cpu)) / np.max(np.abs(u_cpu)))}
         print(f"[bench] cpu baseline {cres['value']:.1f} sample-steps/s on {threads} threads; "
               f"parity {out['parity_vs_cpu']['rel_max_err']:.2e}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
+
+def run_copy(args):
+    """Calibration: ``dst = src * 1`` over ``--copy-bytes`` (reads and writes that many bytes)."""
+    import torch
+    n = args.copy_bytes // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 5}),
+          flush=True)
+
+
+def run_batch(args):
+    """Uneven 64-tube ensemble: batched persistent launches vs per-mesh launches."""
+    import torch
+    from cvfes_amd.batch import BatchedEnsemble
+    from cvfes_amd.configs import batch_cases
+    from cvfes_amd.gmrf import sample_element_E
+    from cvfes_amd.system import MembraneSystem
+
+    steps = 297 if args.steps is None else args.steps
+    block = 99
+    if steps % block:
+        raise SystemExit("--steps must be a multiple of 99 (graph blocks of 99 steps)")
+    ref_steps = max(33, steps // 3)
+    t0 = time.perf_counter()
+    cases = batch_cases(byte_budget=150e9)
+    systems, loads, dts = [], [], []
+    for i, c in enumerate(cases):
+        nodes, el, bdy = c.geometry()
+        E = sample_element_E(el, c.n_smp, seed=c.seed, device="cuda").E
+        s = MembraneSystem(nodes, el, bdy, c.n_smp, vthickness=np.full(nodes.shape[0], c.thickness),
+                           elmE=E, density=c.density, nu=c.nu, store_values=True)
+        dt = 0.8 * s.critical_dt(iters=30, method="matfree")[0]
+        systems.append(s)
+        dts.append(dt)
+        loads.append(s.loads(c.pressures(0, 2 * steps + 2 * ref_steps + 200, dt)))
+        if i % 16 == 0:
+            print(f"[bench] batch mesh {i}: N={c.n_nodes} S={c.n_smp} ({time.perf_counter() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
+    ens = BatchedEnsemble(systems, loads, dts=dts, methods="matfree")
+    ens.initial_state()
+    tuned = ens.autotune()
+    print(f"[bench] batch setup {time.perf_counter() - t0:.0f} s, tasks {ens.n_tasks}, tuned {tuned}",
+          file=sys.stderr, flush=True)
+
+    def timed(fn, n):
+        fn(n)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        fn(n)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / n
+
+    sum_s = sum(c.n_smp for c in cases)
+    t_batch = timed(lambda n: ens.run(n, graph=True, block=block), steps)
+    t_plain = timed(lambda n: ens.run_per_mesh(n, method="stored"), ref_steps)
+    t_mf = timed(lambda n: ens.run_per_mesh(n, method="matfree"), ref_steps)
+    # parity: batched (matrix-free) against per-mesh plain (stored) launches after the same steps
+    ens.initial_state()
+    ens.run(2 * block, graph=True, block=block)
+    batched = [s.u.cpu().numpy() for s in systems]
+    ens.initial_state()
+    ens.run_per_mesh(2 * block, method="stored")
+    worst = max(float(np.max(np.abs(a - s.u.cpu().numpy())) / np.max(np.abs(s.u.cpu().numpy())))
+                for a, s in zip(batched, systems))
+    out = {
+        "metric": "ensemble sample-steps/s, 64 uneven tubes in one persistent launch per step",
+        "value": sum_s / t_batch, "unit": "sample-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": steps, "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded tubes and GMRF samples",
+        "config": {"workload": "batch-64-uneven-tubes", "meshes": len(cases), "sum_samples": sum_s,
+                   "node_samples": int(sum(c.n_smp * c.n_nodes for c in cases)),
+                   "tasks": ens.n_tasks, "grid_waves": tuned["n_waves"], "graph_block": block,
+                   "operator": "matfree"},
+        "per_mesh_plain_ms": 1e3 * t_plain, "per_mesh_matfree_ms": 1e3 * t_mf,
+        "speedup_vs_per_mesh_plain": t_plain / t_batch,
+        "speedup_vs_per_mesh_matfree": t_mf / t_batch,
+        "parity_vs_per_mesh_plain": {"steps": 2 * block, "rel_max_err": worst},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_adjoint(args):
+    """Forward and adjoint gradient of the displacement misfit on the 49,152-node tube."""
+    import torch
+    from cvfes_amd.adjoint import AdjointProblem
+    from cvfes_amd.configs import get_case
+    from cvfes_amd.gmrf import sample_element_E
+    from cvfes_amd.system import MembraneSystem
+
+    case = get_case(args.case if args.case != "tube-256x400-s128" else "tube-192x256-s32")
+    nodes, el, bdy = case.geometry()
+    N, S = nodes.shape[0], case.n_smp
+    theta = np.log(case.sample_E(el).E)
+    sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=np.exp(theta),
+                          density=case.density, nu=case.nu)
+    nsteps, dt = case.nsteps, case.dt
+    load = sysm.loads(case.pressures(0, nsteps))
+    obs = list(range(nsteps // 20, nsteps + 1, nsteps // 20))
+    theta_obs = np.log(sample_element_E(el, S, seed=case.seed + 100).E)
+    sysm.set_log_modulus(torch.as_tensor(theta_obs).cuda())
+    u0 = np.zeros(3 * N)
+    sysm.initial_state(u0, u0, load, dt)
+    states, done = [], 0
+    for k in obs:
+        sysm.step(load, dt, offset=done, nsteps=k - done)
+        done = k
+        states.append(sysm.u.clone())
+    prob = AdjointProblem(sysm, load, nsteps, dt, method=args.method, obs_steps=obs,
+                          u_obs=torch.stack(states), checkpoint_every=100)
+    prob.set_theta(torch.as_tensor(theta).cuda())
+    prob.forward()
+    prob.backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prob.forward()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g = prob.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, adj = t1 - t0, t2 - t1
+    out = {
+        "metric": "adjoint gradient time / forward solve time (49k-node tube, 32 samples, 2,000 steps)",
+        "value": adj / fwd, "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": nsteps,
+        "ms_per_step": 1e3 * (fwd + adj) / nsteps, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded tube and GMRF samples",
+        "config": {"workload": case.name, "n_nodes": N, "n_samples": S, "observations": len(obs),
+                   "checkpoint_every": 100, "operator": args.method},
+        "forward_s": fwd, "adjoint_s": adj,
+        "forward_sample_steps_per_s": S * nsteps / fwd,
+        "gradient_max_abs": float(g.abs().max()),
+    }
     print(json.dumps(out), flush=True)
 
 

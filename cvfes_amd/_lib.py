@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -56,6 +56,18 @@ class CvfesLoad(ctypes.Structure):
     _fields_ = [("p", _vp), ("ga", _vp), ("qa", _vp), ("gb", _vp), ("qb", _vp)]
 
 
+class CvfesGeometry(ctypes.Structure):
+    """Mirror of ``cvfes_geometry_t``."""
+
+    _fields_ = [("nodes", _vp), ("elems", _vp), ("nu", _f64), ("kshear", _f64)]
+
+
+class CvfesUpdate(ctypes.Structure):
+    """Mirror of ``cvfes_update_t``."""
+
+    _fields_ = [("interval", _i64), ("counter", _i64), ("has_geometry", _i64), ("ug", _vp)]
+
+
 class CvfesBatchMesh(ctypes.Structure):
     """Mirror of ``cvfes_batch_mesh_t``."""
 
@@ -82,6 +94,11 @@ _SIGNATURES = {
                                   _vp, _f64, _f64, _f64, _int]),
     "cvfes_grad_accum": (_int, [_vp, ctypes.POINTER(CvfesSystem), _vp, _vp, _vp, _vp, _vp, _f64,
                                 _f64, _f64, _vp]),
+    "cvfes_element_stress": (_int, [_vp, _i64, _i64, ctypes.POINTER(CvfesGeometry), _vp, _vp, _vp,
+                                    _vp]),
+    "cvfes_step_updated": (_int, [_vp, ctypes.POINTER(CvfesSystem), ctypes.POINTER(CvfesGeometry),
+                                  ctypes.POINTER(CvfesUpdate), ctypes.POINTER(_vp),
+                                  ctypes.POINTER(CvfesLoad), _i64, _f64, _f64, _f64]),
     "cvfes_batch_plan": (_int, [_vp, ctypes.POINTER(CvfesBatchMesh), _int, _vp,
                                 ctypes.POINTER(_i64)]),
     "cvfes_batch_step": (_int, [_vp, _vp, _int, _i64, _int, _vp, _i64, _i64]),

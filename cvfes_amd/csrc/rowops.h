@@ -7,6 +7,37 @@
 
 namespace cvfes {
 
+// Lanes per block row.  Packed (UNIFORM = false): S/VW lanes, so a wave may span several rows.
+// Wave-uniform (UNIFORM = true): S/VW rounded up to whole wavefronts; lanes past the last sample
+// are inactive (they read the last sample group and do not store), which keeps every row on
+// whole wavefronts for any S.
+template <bool UNIFORM, int VW>
+__host__ __device__ __forceinline__ int64_t lanes_per_row(int64_t n_smp) {
+    const int64_t l = n_smp / VW;
+    return UNIFORM ? (l + 63) / 64 * 64 : l;
+}
+
+template <bool UNIFORM, int VW>
+__device__ __forceinline__ bool row_and_sample(int64_t n_nodes, int64_t n_smp, int32_t &row,
+                                               int64_t &s, bool &active) {
+    const int64_t lpr = lanes_per_row<UNIFORM, VW>(n_smp);
+    const int64_t g = xcd_block() * kBlock + threadIdx.x;
+    int64_t r = g / lpr;
+    s = (g - r * lpr) * VW;
+    active = s < n_smp;
+    if (!active) s = n_smp - VW;
+    if constexpr (UNIFORM) r = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(r));
+    row = static_cast<int32_t>(r);
+    return r < n_nodes;
+}
+
+// Wave-uniform rows pay off when they keep at least 3/4 of the lanes busy.
+template <int VW>
+inline bool use_uniform(int64_t n_smp) {
+    const int64_t l = n_smp / VW, padded = (l + 63) / 64 * 64;
+    return 4 * l >= 3 * padded;
+}
+
 struct RowArgs {
     int64_t n_nodes;
     int64_t n_smp;

@@ -15,7 +15,13 @@ kernels it replaces:
   optional Rayleigh ``beta``), traction and Dirichlet values are one kernel per step on the GPU,
   with no host round trip; ``Run`` advances many steps in one call;
 * the lumped mass follows the CPU path (``mass_lumping = cpu``, oracle parity) unless
-  ``mass_lumping = gpu`` selects the ``/3`` form of the OpenCL kernels.
+  ``mass_lumping = gpu`` selects the ``/3`` form of the OpenCL kernels;
+* with ``stiffness_update_interval = k`` in the configuration (``config.update_interval``), the
+  stiffness and the pressure load are those of the moved geometry ``X + u``, re-formed every ``k``
+  steps as the reference CPU path does (``physicsSolver.py:294-324``); ``k = 1`` is the
+  every-step ``K(X+u)`` of ``assemble_K_P``.  This mode is matrix-free;
+* ``CalculateStress`` / ``SaveStress`` give the per-element stresses of
+  ``SolidSolver.Save`` (``physicsSolver.py:384-424``) from a GPU kernel.
 """
 from __future__ import annotations
 
@@ -67,6 +73,8 @@ class GPUSolidSolver(PhysicsSolver):
         self.damp = float(mesh.damp) if self.damped else 0.0
         self.beta = float(getattr(config, "beta", 0.0) or 0.0)
         self.method = getattr(config, "stiffness_method", "stored") or "stored"
+        interval = getattr(config, "update_interval", None)
+        self.updateInterval = int(interval) if interval is not None and int(interval) < (1 << 40) else None
         self.massLumping = getattr(config, "mass_lumping", "cpu") or "cpu"
         self.time0 = float(getattr(config, "time", 0.0) or 0.0)
 
@@ -110,7 +118,8 @@ class GPUSolidSolver(PhysicsSolver):
             np.asarray(mesh.nodes)[self.lclNodeIds], mesh.lclElmNodeIds, self.lclBoundary, self.nSmp,
             vthickness=mesh.vthickness[self.lclNodeIds], vE=None if elmE is not None else mesh.vE[self.lclNodeIds],
             elmE=elmE, density=mesh.density, nu=mesh.v, mass_form=self.massLumping,
-            bdy_u=float(mesh.bdyU), store_values=(self.method == "stored"), device=self.device)
+            bdy_u=float(mesh.bdyU), store_values=(self.method == "stored" and self.updateInterval is None),
+            device=self.device)
         self.t = self.time0
         load0 = self._load_tables(np.array([self.time0]))
         self.system.initial_state(self.u, self.du, load0[0][1], self.dt, beta=self.beta,
@@ -162,15 +171,21 @@ class GPUSolidSolver(PhysicsSolver):
         runs = self._load_tables(np.array([self.t]))
         n, tables = runs[0]
         tables.p.fill_(self.pressure)
-        self.system.step(tables, dt, beta=self.beta, damp=self.damp, method=self.method)
+        self._advance(tables, dt)
         self._step_load = tables
+
+    def _advance(self, tables, dt):
+        if self.updateInterval is None:
+            self.system.step(tables, dt, beta=self.beta, damp=self.damp, method=self.method)
+        else:
+            self.system.step_updated(tables, dt, self.updateInterval, beta=self.beta, damp=self.damp)
 
     def Run(self, t0, nsteps, dt):
         """``nsteps`` steps from ``t0`` in as few native calls as the traction series allows;
         step ``n`` uses the load at ``t0 + n dt`` (no host round trip between steps)."""
         times = t0 + np.arange(int(nsteps), dtype=np.float64) * dt
         for n, tables in self._load_tables(times):
-            self.system.step(tables, dt, beta=self.beta, damp=self.damp, method=self.method)
+            self._advance(tables, dt)
         self.t = float(t0 + nsteps * dt)
         return self.system.u
 
@@ -178,6 +193,21 @@ class GPUSolidSolver(PhysicsSolver):
     def GetDisplacement(self):
         """Current displacement as the reference's ``[ndof, nSmp]`` host array."""
         return self.system.u.cpu().numpy().reshape(self.lclNDof, self.nSmp)
+
+    def CalculateStress(self):
+        """Per-element stresses ``[nElms, nSmp, 5]`` (xx, yy, xy, xz, yz) of the current
+        displacement on the current geometry (``physicsSolver.py:416-424``)."""
+        return self.system.element_stress().permute(0, 2, 1).cpu().numpy()
+
+    def SaveStress(self, filename, counter):
+        """Write displacements and per-element stresses, as ``SolidSolver.Save``
+        (``physicsSolver.py:384-398``; arrays ``displacement_###`` and ``xx_###`` ...)."""
+        resU = self.UnionDisplacement(self.GetDisplacement())
+        stress = self.CalculateStress()
+        if self.rank == 0:
+            u = resU.transpose().reshape(self.nSmp, self.mesh.nNodes, self.Dof)
+            vtkio.save_solution(filename, counter, self.mesh.nodes, self.mesh.elementNodeIds, u, stress)
+        self.comm.Barrier()
 
     def ApplyBoundaryCondition(self, quant):
         quant[self.bdyDofs, :] = self.mesh.bdyU

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import CvfesLoad, CvfesSystem, check, ptr
+from ._lib import CvfesGeometry, CvfesLoad, CvfesSystem, CvfesUpdate, check, ptr
 from .sparse import BlockMaps, block_structure
 
 __all__ = ["MembraneSystem", "LoadTables", "MASS_SCALE"]
@@ -140,6 +140,16 @@ class MembraneSystem:
                 vthick = vthick[:, 0]
             self.thick_sstride = 1 if vthick.ndim == 2 else 0
             self.vthick = _dev(vthick, f64, dev)
+            # element-average modulus for the stress recovery (mean over the Gauss points,
+            # physicsSolver.py:416-424)
+            self.ave_E = None
+            if coef is None and elmE is not None:
+                self.ave_E = _dev(np.asarray(elmE, np.float64).reshape(self.n_elms, S), f64, dev)
+            elif coef is None and vE is not None:
+                vEa = np.asarray(vE, np.float64).reshape(self.n_nodes, S)
+                gE = (0.5 * (vEa[elements[:, 0]] + vEa[elements[:, 1]]) + 0.5 * (vEa[elements[:, 0]] + vEa[elements[:, 2]])
+                      + 0.5 * (vEa[elements[:, 1]] + vEa[elements[:, 2]])) / 3.0
+                self.ave_E = _dev(gE, f64, dev)
             if coef is not None:
                 self.coef = _dev(np.asarray(coef, np.float64).reshape(self.n_elms, S), f64, dev)
             else:
@@ -167,6 +177,8 @@ class MembraneSystem:
                 self.assemble_values()
         self._sys = self._make_struct()
         self.state = None
+        self.update = None
+        self.ug = None
 
     # ------------------------------------------------------------------------------------------
     def set_log_modulus(self, theta):
@@ -178,6 +190,7 @@ class MembraneSystem:
         check(self.lib.cvfes_element_coef(self.stream, self.n_elms, self.n_smp, ptr(self.elems),
                                           ptr(self.vthick), self.thick_sstride, None, ptr(theta), 1,
                                           ptr(self.coef)))
+        self.ave_E = torch.exp(theta)
         if self.values is not None:
             self.assemble_values()
 
@@ -226,6 +239,59 @@ class MembraneSystem:
         return (self.n_nodes, 3, self.n_smp)
 
     # ------------------------------------------------------------------------------------------
+    def geometry(self):
+        return CvfesGeometry(nodes=ptr(self.nodes), elems=ptr(self.elems), nu=self.nu, kshear=self.kshear)
+
+    def element_stress(self, u=None, ug=None, ave_E=None):
+        """Per-element stress ``[E, 5, S]`` (xx, yy, xy, xz, yz) of ``u`` (default: the current
+        displacement) on the geometry ``X + ug`` (default: the reference geometry, or in the
+        updated-geometry mode the geometry of the last re-formation, as ``mesh.updateNodes``)."""
+        u = self.u if u is None else u.contiguous()
+        if ug is None and self.update is not None and self.update.has_geometry:
+            # geometry of the last re-formation: with interval 1 the displacement the last step
+            # started from (u_{n-1}), otherwise the saved copy
+            ug = self.state[1] if self.update.interval == 1 else self.ug
+        ave_E = self.ave_E if ave_E is None else ave_E.contiguous()
+        if ave_E is None:
+            raise ValueError("the system was built from coef: pass ave_E")
+        out = torch.empty((self.n_elms, 5, self.n_smp), dtype=torch.float64, device=self.device)
+        geo = self.geometry()
+        check(self.lib.cvfes_element_stress(self.stream, self.n_elms, self.n_smp, ctypes.byref(geo),
+                                            ptr(u), ptr(ug), ptr(ave_E), ptr(out)))
+        return out
+
+    def step_updated(self, load, dt, interval, beta=0.0, damp=0.0, offset=0, nsteps=None):
+        """Advance ``nsteps`` steps of the updated-geometry mode (stiffness and unit-traction load
+        re-formed on ``X + u`` every ``interval`` steps, matrix-free); the schedule continues
+        across calls until the next :meth:`initial_state`."""
+        if self.state is None:
+            raise RuntimeError("call initial_state() first")
+        if not isinstance(load, LoadTables):
+            load = self.loads(load)
+        nsteps = load.n - offset if nsteps is None else int(nsteps)
+        if offset < 0 or nsteps < 0 or offset + nsteps > load.n:
+            raise ValueError("steps exceed the load tables")
+        interval = int(interval)
+        if interval < 1:
+            raise ValueError("interval must be >= 1")
+        if self.update is None:
+            self.update = CvfesUpdate(interval=interval, counter=0, has_geometry=0, ug=None)
+        elif self.update.interval != interval:
+            raise ValueError("the re-formation interval is fixed until the next initial_state()")
+        if interval > 1 and self.ug is None:
+            self.ug = torch.empty(self.state_shape(), dtype=torch.float64, device=self.device)
+        self.update.ug = ptr(self.ug)
+        arr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in self.state])
+        ls = load.struct(offset)
+        geo = self.geometry()
+        check(self.lib.cvfes_step_updated(self.stream, ctypes.byref(self._sys), ctypes.byref(geo),
+                                          ctypes.byref(self.update), arr, ctypes.byref(ls),
+                                          int(nsteps), float(dt), float(beta), float(damp)))
+        by_ptr = {b.data_ptr(): b for b in self.state}
+        self.state = [by_ptr[arr[i]] for i in range(3)]
+        self._load_keepalive = load
+        return self.state[0]
+
     def spmv(self, x, method="stored", out=None):
         """``y = K x`` for ``x [N,3,S]`` (device tensor)."""
         x = x.contiguous()
@@ -258,6 +324,7 @@ class MembraneSystem:
                                            ptr(bufs[2])))
         torch.cuda.current_stream(self.device).synchronize()
         self.state = bufs
+        self.update = None
         return bufs
 
     def step(self, load, dt, beta=0.0, damp=0.0, method="stored", offset=0, nsteps=None):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 1
+#define CVFES_ABI_VERSION 2
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -200,6 +200,54 @@ int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int method, cons
 int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const int32_t *elems,
                      const int32_t *cpos, const double *mu, const double *x_cur,
                      const double *x_prev, double dt, double beta, double nu_scale, double *grad);
+
+/*
+ * Reference geometry of a mesh for the kernels that work on the moved geometry X + u.
+ */
+typedef struct cvfes_geometry {
+    const double *nodes;    /* [N][3] reference coordinates                                      */
+    const int32_t *elems;   /* [E][3] element node ids                                           */
+    double nu;              /* Poisson ratio                                                     */
+    double kshear;          /* transverse-shear factor (5/6)                                     */
+} cvfes_geometry_t;
+
+/*
+ * Per-element stress of the displacement u ([N][3][S]) on the geometry X + ug (ug [N][3][S], or
+ * NULL for the reference geometry), with the element-average Young's modulus ave_E [E][S]:
+ * stress[e][c][s], c = (xx, yy, xy, xz, yz) in global axes.
+ * Replaces: OptimizedCalculateStress (optimizedSolidAssemble.pyx:1116-1223) as called by
+ *   SolidSolver.OptimizedCalculateStress (physicsSolver.py:416-424, ug = mesh.updateNodes - X);
+ *   calc_stress (physicsSolverGPUs.cl:443-589).
+ */
+int cvfes_element_stress(void *stream, int64_t n_elms, int64_t n_smp, const cvfes_geometry_t *geo,
+                         const double *u, const double *ug, const double *ave_E, double *stress);
+
+/*
+ * Re-formation schedule of the updated-geometry mode (host memory, updated by each call):
+ * before every step, if counter >= interval the geometry becomes X + (current u) and counter is
+ * set to 1, otherwise counter is incremented.  counter starts at 0, so the first re-formation
+ * comes before step `interval` (0-based), as in the reference.  ug: device [N][3][S] buffer for
+ * the geometry displacement (needed when interval > 1; with interval 1 the current u is used).
+ */
+typedef struct cvfes_update {
+    int64_t interval;
+    int64_t counter;
+    int64_t has_geometry;   /* 0 until the first re-formation (reference geometry until then)   */
+    double *ug;
+} cvfes_update_t;
+
+/*
+ * nsteps steps of the updated-geometry mode: as cvfes_step, with the stiffness and the
+ * unit-traction load those of the geometry X + ug (never stored: every step forms each element's
+ * frame on the moved geometry in registers); lumped mass, coefficients and traction fields stay
+ * as assembled.  Needs sys->nptr/nent/coef/lm.
+ * Replaces: SolidSolver.Solve with OptimizedAssembleUpdate (physicsSolver.py:263-324;
+ *   optimizedSolidAssemble.pyx:710-819) for stiffness_update_interval = interval; the K(X+u) and
+ *   follower-load products of assemble_K_P (physicsSolverSDampGPUs.cl:232-391) for interval 1.
+ */
+int cvfes_step_updated(void *stream, const cvfes_system_t *sys, const cvfes_geometry_t *geo,
+                       cvfes_update_t *upd, double **state, const cvfes_load_t *load,
+                       int64_t nsteps, double dt, double beta, double damp);
 
 /*
  * Batched ensemble (no counterpart in the reference, which advances one mesh per process): many

@@ -467,6 +467,11 @@ def run_updated(nodes, elements, D, coefK, indptr, indices, K0, LM, f1, pressure
     return u, up, snaps, n_up
 
 
+def _to_global(T, St):
+    """``T^T S T``: local stress tensor to global axes (``pyx:1207-1208``)."""
+    return np.einsum("eski,eskl,eslj->esij", T, St, T)
+
+
 def calculate_stress(nodes_s, elements, D, ave_E, u):
     """``OptimizedCalculateStress`` (``optimizedSolidAssemble.pyx:1116-1223``): global stress
     components ``(xx, yy, xy, xz, yz)`` ``[E, S, 5]`` of the displacement ``u [ndof, S]`` on the
@@ -490,5 +495,5 @@ def calculate_stress(nodes_s, elements, D, ave_E, u):
     St[..., 0, 1] = St[..., 1, 0] = ls[..., 2]
     St[..., 0, 2] = St[..., 2, 0] = ls[..., 3]
     St[..., 1, 2] = St[..., 2, 1] = ls[..., 4]
-    G = np.einsum("eski,eskl,eslj->esij", T, St, T)           # T^T S T
+    G = _to_global(T, St)
     return np.stack([G[..., 0, 0], G[..., 1, 1], G[..., 0, 1], G[..., 0, 2], G[..., 1, 2]], axis=-1)

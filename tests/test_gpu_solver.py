@@ -3,7 +3,9 @@
 * ``TransientSolverGPU`` per-step loop (fluid time rounded to 4 decimals, ``solver.py:118``);
 * the fast ``Run`` path (exact step times);
 * fluid wall-traction input (snapshot series and constant stress) and the legacy module;
-* ``Save`` writes the per-sample displacement arrays.
+* ``Save`` writes the per-sample displacement arrays;
+* ``stiffness_update_interval`` selects the updated-geometry mode (oracle ``run_updated``), and
+  ``SaveStress`` writes the oracle's per-element stresses.
 Bar: 1e-10 relative max-norm on the displacements.
 """
 import os
@@ -159,6 +161,42 @@ def test_save_writes_per_sample_displacements(tmp_path, driver):
     u = solver.solidSolver.GetDisplacement()
     assert len([k for k in back.point_data if k.startswith("u_")]) == 16
     assert np.array_equal(back.point_data["u_003"].ravel(), u[:, 3])
+
+
+def test_updated_geometry_solver_and_stress(tmp_path, driver):
+    from cvfes_amd import vtkio
+    cfg = write_cfg(tmp_path, endtime=endtime(120), dt=repr(DT), stiffness_method="matfree",
+                    stiffness_update_interval="2")
+    cv = driver().ReadInputFile(cfg)
+    solver = cv.Solve(fast=True)
+    mesh = cv.meshes["wall"]
+    wf = solver.solidSolver.waveform
+    p = wf(np.arange(120) * DT)
+    om = OracleModel(mesh)
+    D = so.d_matrix(mesh.v)
+    _, gth = so.gauss_properties(mesh.vE, mesh.vthickness, mesh.elementNodeIds)
+    coefK = so.coef_k_element_E(gth, mesh.elmE)
+    z = np.zeros(3 * mesh.nNodes)
+    u, _, _, nup = so.run_updated(mesh.nodes, mesh.elementNodeIds, D, coefK, om.indptr, om.indices,
+                                  om.K, om.LM, om.f1, p, z, z, DT, 2, bdy_dofs=om.bdy_dofs)
+    assert nup == 59
+    ug = solver.solidSolver.GetDisplacement()
+    assert rel_err(ug, u) < REL
+    # stresses on the geometry of the last re-formation (the displacement before step 118)
+    _, _, snaps, _ = so.run_updated(mesh.nodes, mesh.elementNodeIds, D, coefK, om.indptr, om.indices,
+                                    om.K, om.LM, om.f1, p[:118], z, z, DT, 2, bdy_dofs=om.bdy_dofs,
+                                    record=(118,))
+    S = mesh.nSmp
+    moved = mesh.nodes[None] + snaps[118].T.reshape(S, mesh.nNodes, 3)
+    ref = so.calculate_stress(moved, mesh.elementNodeIds, D, mesh.elmE, ug)
+    st = solver.solidSolver.CalculateStress()
+    assert rel_err(st, ref) < 1e-9
+    out = str(tmp_path / "wall.vtp")
+    solver.solidSolver.SaveStress(out, 7)
+    back = vtkio.read_vtp(str(tmp_path / "wall7.vtp"))
+    assert np.array_equal(back.cell_data["xz_005"].ravel(), st[:, 5, 3])
+    assert np.array_equal(back.point_data["displacement_002"].reshape(-1, 3),
+                          ug[:, 2].reshape(-1, 3))
 
 
 def test_refuses_without_library(monkeypatch, tmp_path):

@@ -379,3 +379,158 @@ def test_c_oracle_matches_numpy_oracle():
     a = so.step(ip, ix, K, LM, LM, 1e4 * f, u, up, 1e-7, beta=1e-5, damp=3.0, bdy_dofs=bd)
     b = co.step(ip, ix, K, LM, LM, f, 1e4, u, up, 1e-7, beta=1e-5, damp=3.0, bdy_dofs=bd, threads=2)
     assert np.allclose(a, b, rtol=0, atol=1e-14 * np.abs(a).max())
+
+
+# ----------------------------------------------------------------------------------------------
+# Updated geometry and stress
+# ----------------------------------------------------------------------------------------------
+
+
+def moved_tube(S=3, amp=0.02, seed=4):
+    nodes, el, bdy = tube_geometry(10, 5, 0.4, 1.5)
+    n = nodes.shape[0]
+    rng = np.random.default_rng(seed)
+    u = amp * rng.standard_normal((3 * n, S))
+    coefK = 0.07 * 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
+    ip, ix = so.sparse_info_loops(el, n)
+    return nodes, el, u, coefK, ip, ix
+
+
+def test_update_assembly_equals_reference_on_moved_mesh():
+    nodes, el, u, coefK, ip, ix = moved_tube()
+    S, n = u.shape[1], nodes.shape[0]
+    moved = nodes[None] + u.T.reshape(S, n, 3)
+    D = so.d_matrix(0.45)
+    K, f = so.assemble_update(moved, el, 1.7, D, coefK, ip, ix)
+    gth = np.ones((el.shape[0], 1, 3))
+    for s in range(S):
+        _, Ks, fs, _ = so.assemble_reference(moved[s], el, 1.7, D, coefK[:, s:s + 1], gth, 1.0, ip, ix)
+        assert np.abs(K[:, s] - Ks[:, 0]).max() <= 1e-13 * np.abs(Ks).max()
+        assert np.abs(f[:, s] - fs).max() <= 1e-13 * np.abs(fs).max()
+    # planted mistake: every sample on the geometry of sample 0
+    Kbad, _ = so.assemble_update(np.repeat(moved[:1], S, axis=0), el, 1.7, D, coefK, ip, ix)
+    assert np.abs(Kbad[:, 1] - K[:, 1]).max() > 1e-6 * np.abs(K).max()
+
+
+def test_update_assembly_rigid_motions():
+    nodes, el, u, coefK, ip, ix = moved_tube(S=2)
+    D = so.d_matrix(0.5)
+    base = np.repeat(nodes[None], 2, axis=0)
+    K0, f0 = so.assemble_update(base, el, 1.0, D, coefK, ip, ix)
+    Kt, ft = so.assemble_update(base + np.array([0.3, -1.1, 2.5]), el, 1.0, D, coefK, ip, ix)
+    assert np.abs(Kt - K0).max() <= 1e-11 * np.abs(K0).max()
+    assert np.abs(ft - f0).max() <= 1e-11 * np.abs(f0).max()
+    c, s_ = math.cos(0.7), math.sin(0.7)
+    R = np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]]) @ np.array(
+        [[1.0, 0.0, 0.0], [0.0, math.cos(0.3), -math.sin(0.3)], [0.0, math.sin(0.3), math.cos(0.3)]])
+    Kr, fr = so.assemble_update(base @ R.T, el, 1.0, D, coefK, ip, ix)
+    rot = np.einsum("ik,pskl,jl->psij", R, K0, R)                 # R K R^T
+    assert np.abs(Kr - rot).max() <= 1e-11 * np.abs(K0).max()
+    fr_exp = np.einsum("ij,njs->nis", R, f0.reshape(-1, 3, 2)).reshape(-1, 2)
+    assert np.abs(fr - fr_exp).max() <= 1e-11 * np.abs(f0).max()
+
+
+def test_run_updated_schedule_and_linear_limit():
+    nodes, el, bdy = tube_geometry(8, 4, 0.3, 1.0)
+    n = nodes.shape[0]
+    S = 2
+    rng = np.random.default_rng(1)
+    coefK = 0.07 * 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
+    ip, ix = so.sparse_info_loops(el, n)
+    D = so.d_matrix(0.5)
+    gth = np.full((el.shape[0], S, 3), 0.07)
+    _, K0, f1, LM = so.assemble_reference(nodes, el, 1.0, D, coefK, gth, 1.0, ip, ix)
+    bd = (3 * bdy[:, None] + np.arange(3)).ravel()
+    p = 1.33e4 * np.ones(10)
+    z = np.zeros(3 * n)
+    dt = 2e-6
+    ul, upl, _ = so.run(ip, ix, K0, LM, f1, p, z, z, dt, beta=1e-7, bdy_dofs=bd)
+    u, up, _, nup = so.run_updated(nodes, el, D, coefK, ip, ix, K0, LM, f1, p, z, z, dt, 11,
+                                   beta=1e-7, bdy_dofs=bd)
+    assert nup == 0 and np.array_equal(u, ul) and np.array_equal(up, upl)
+    for interval, expect in ((3, 3), (1, 9), (10, 0), (9, 1)):
+        u, _, _, nup = so.run_updated(nodes, el, D, coefK, ip, ix, K0, LM, f1, p, z, z, dt,
+                                      interval, beta=1e-7, bdy_dofs=bd)
+        assert nup == expect
+    # the moved geometry changes the answer, slightly
+    u1, _, _, _ = so.run_updated(nodes, el, D, coefK, ip, ix, K0, LM, f1, p, z, z, dt, 1,
+                                 beta=1e-7, bdy_dofs=bd)
+    d = np.abs(u1 - ul).max() / np.abs(ul).max()
+    assert 0 < d < 1e-2
+
+
+def flat_patch(nx=5, ny=4):
+    xs, ys = np.meshgrid(np.linspace(0, 1, nx), np.linspace(0, 0.7, ny), indexing="ij")
+    P = np.stack([xs.ravel(), ys.ravel(), np.zeros(nx * ny)], axis=1)
+    idx = np.arange(nx * ny).reshape(nx, ny)
+    el = []
+    for i in range(nx - 1):
+        for j in range(ny - 1):
+            el += [[idx[i, j], idx[i + 1, j], idx[i + 1, j + 1]], [idx[i, j], idx[i + 1, j + 1], idx[i, j + 1]]]
+    return P, np.array(el)
+
+
+def test_stress_rigid_translation_is_zero():
+    nodes, el, u, coefK, ip, ix = moved_tube(S=2)
+    n = nodes.shape[0]
+    uc = np.tile(np.array([0.2, -0.4, 0.9]), n)[:, None] * np.ones((1, 2))
+    st = so.calculate_stress(np.repeat(nodes[None], 2, axis=0), el, so.d_matrix(0.4),
+                             np.full((el.shape[0], 2), 7e6), uc)
+    scale = 7e6 * 0.9 / 0.1
+    assert np.abs(st).max() <= 1e-12 * scale
+
+
+def test_stress_homogeneous_plane_strain():
+    P, el = flat_patch()
+    c5, s5 = math.cos(0.5), math.sin(0.5)            # edges off the axes: frames at odd angles
+    P[:, :2] = P[:, :2] @ np.array([[c5, s5], [-s5, c5]])
+    nu, E = 0.3, 5e6
+    eps = np.array([[1e-3, 4e-4], [4e-4, -6e-4]])
+    u = np.zeros((P.shape[0], 3))
+    u[:, :2] = P[:, :2] @ eps.T
+    st = so.calculate_stress(P[None], el, so.d_matrix(nu), np.full((el.shape[0], 1), E),
+                             u.reshape(-1, 1))[:, 0]
+    c = E / (1 - nu * nu)
+    expect = np.array([c * (eps[0, 0] + nu * eps[1, 1]), c * (eps[1, 1] + nu * eps[0, 0]),
+                       c * (1 - nu) * eps[0, 1], 0.0, 0.0])
+    assert np.abs(st - expect).max() <= 1e-10 * np.abs(expect).max()
+    # planted mistake: rotating the local tensor the wrong way (T S T^T) breaks the closed form
+    T = so.coordinate_transformation(P, el)
+    assert len({tuple(np.round(t[0], 6)) for t in T}) > 1       # element frames differ
+    good = so._to_global
+    try:
+        so._to_global = lambda T, St: good(np.swapaxes(T, -1, -2), St)
+        bad = so.calculate_stress(P[None], el, so.d_matrix(nu), np.full((el.shape[0], 1), E),
+                                  u.reshape(-1, 1))[:, 0]
+    finally:
+        so._to_global = good
+    assert np.abs(bad - expect).max() > 1e-3 * np.abs(expect).max()
+
+
+def test_stress_nodal_force_equals_element_stiffness_product():
+    # h A TT^T B^T sigma_local = K_e u_e, with sigma_local rotated back from the global tensor; the
+    # dropped global zz entry follows from the local zz stress being zero (tilted patch: e3_z != 0)
+    P, el = flat_patch()
+    P[:, 2] = 0.3 * P[:, 0] + 0.2 * P[:, 1]
+    rng = np.random.default_rng(8)
+    u = 1e-3 * rng.standard_normal((3 * P.shape[0], 1))
+    nu, h = 0.45, 0.07
+    E = 7e6 * np.exp(0.1 * rng.standard_normal(el.shape[0]))
+    st = so.calculate_stress(P[None], el, so.d_matrix(nu), E[:, None], u)[:, 0]
+    Ke, area, _ = so.element_blocks_closed_form(P, el, nu)
+    ue = u[(3 * el[:, :, None] + np.arange(3)).reshape(-1, 9), 0]
+    fk = np.einsum("eij,ej->ei", Ke * (h * E)[:, None, None], ue)
+    T = so.coordinate_transformation(P, el)
+    _, B = so._local_geometry(P, el, T)
+    G = np.zeros((el.shape[0], 3, 3))
+    G[:, 0, 0], G[:, 1, 1] = st[:, 0], st[:, 1]
+    G[:, 0, 1] = G[:, 1, 0] = st[:, 2]
+    G[:, 0, 2] = G[:, 2, 0] = st[:, 3]
+    G[:, 1, 2] = G[:, 2, 1] = st[:, 4]
+    rest = np.einsum("ek,ekl,el->e", T[:, 2], G, T[:, 2])
+    G[:, 2, 2] = -rest / T[:, 2, 2] ** 2
+    L = np.einsum("eik,ekl,ejl->eij", T, G, T)                    # T G T^T: local tensor
+    sig = np.stack([L[:, 0, 0], L[:, 1, 1], L[:, 0, 1], L[:, 0, 2], L[:, 1, 2]], axis=1)
+    fl = h * area[:, None] * np.einsum("eki,ek->ei", B, sig)       # local nodal forces
+    fs = np.einsum("eji,ej->ei", so._big_T(T), fl)
+    assert np.abs(fs - fk).max() <= 1e-10 * np.abs(fk).max()
