@@ -19,6 +19,10 @@ Other workloads (``--workload``):
            the matrix-free step;
   adjoint  forward + adjoint gradient of the 49,152-node tube (32 samples, 2,000 steps, 20
            observations, checkpoints every 100 steps) and the adjoint / forward time ratio;
+  combined the stenosed 512,000-node tapered tube with 256 samples and beta = 1e-4: stored and
+           matrix-free steps timed, then 2,000 forward steps and the adjoint gradient with two-level
+           checkpoints (every 200 steps, then every 20 inside a segment) and the observations in
+           host memory, sized to fit 288 GB;
   copy     a plain device copy of ``--copy-bytes`` (the calibration kernel for the counters).
 """
 from __future__ import annotations
@@ -122,7 +126,8 @@ def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "adjoint", "copy"])
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "batch", "adjoint", "combined", "copy"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
@@ -146,6 +151,8 @@ def main():
         return run_adjoint(args)
     if args.workload == "copy":
         return run_copy(args)
+    if args.workload == "combined":
+        return run_combined(args)
     args.steps = 200 if args.steps is None else args.steps
     args.warmup = 20 if args.warmup is None else args.warmup
 
@@ -357,6 +364,91 @@ def run_adjoint(args):
                    "checkpoint_every": 100, "operator": args.method},
         "forward_s": fwd, "adjoint_s": adj,
         "forward_sample_steps_per_s": S * nsteps / fwd,
+        "gradient_max_abs": float(g.abs().max()),
+    }
+    print(json.dumps(out), flush=True)
+
+
+
+def run_combined(args):
+    """Stenosed 512,000-node tube, 256 samples: stored vs matrix-free steps, forward + adjoint."""
+    import torch
+    from cvfes_amd.adjoint import AdjointProblem
+    from cvfes_amd.configs import get_case
+    from cvfes_amd.gmrf import sample_element_E
+    from cvfes_amd.system import MembraneSystem
+
+    log = lambda msg: print(f"[bench] combined {msg}", file=sys.stderr, flush=True)
+    t_start = time.perf_counter()
+    case = get_case("stenosed-512x1000-s256")
+    nodes, el, bdy = case.geometry()
+    N, S, E = nodes.shape[0], case.n_smp, el.shape[0]
+    gm = sample_element_E(el, S, seed=case.seed, E0=case.E0, cov=case.cov, device="cuda")
+    theta = np.log(gm.E)
+    log(f"GMRF samples ({time.perf_counter() - t_start:.0f} s)")
+    sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=gm.E,
+                          density=case.density, nu=case.nu, store_values=True)
+    beta = case.beta
+    dt = case.dt or 0.8 * sysm.critical_dt(beta=beta, iters=40, method="matfree")[0]
+    nsteps = case.nsteps if args.steps is None else args.steps
+    load = sysm.loads(case.pressures(0, nsteps + 100, dt))
+    log(f"assembled N={N} E={E} nnzb={sysm.nnzb} S={S} dt={dt:.4e} ({time.perf_counter() - t_start:.0f} s)")
+    per_step = {}
+    finals = {}
+    for method in ("stored", "matfree"):
+        wall, kern_ms = time_steps(sysm, load, dt, beta, method, 5, 50)
+        per_step[method] = {"kernel_ms": kern_ms, "sample_steps_per_s": S / (wall / 50),
+                            "achieved_GBs": algorithmic_bytes(sysm, method) / (kern_ms * 1e-3) / 1e9,
+                            "algorithmic_bytes_per_launch": algorithmic_bytes(sysm, method)}
+        finals[method] = sysm.u.cpu().numpy()
+        log(f"{method}: {kern_ms:.3f} ms/step")
+    agree = float(np.max(np.abs(finals["stored"] - finals["matfree"])) / np.max(np.abs(finals["stored"])))
+    del finals
+    sysm.release_values()
+    torch.cuda.empty_cache()
+    # observations: the same model under a second GMRF modulus field, kept in pinned host memory
+    theta_obs = np.log(sample_element_E(el, S, seed=case.seed + 100, E0=case.E0, cov=case.cov,
+                                        tau=gm.tau, device="cuda").E)
+    sysm.set_log_modulus(torch.as_tensor(theta_obs).cuda())
+    obs = list(range(nsteps // 20, nsteps + 1, nsteps // 20))
+    u_obs = torch.empty((len(obs), N, 3, S), dtype=torch.float64, pin_memory=True)
+    u0 = np.zeros(3 * N)
+    sysm.initial_state(u0, u0, load, dt, beta=beta, method="matfree")
+    done = 0
+    for i, k in enumerate(obs):
+        sysm.step(load, dt, beta=beta, method="matfree", offset=done, nsteps=k - done)
+        done = k
+        u_obs[i].copy_(sysm.u)
+    log(f"observations ({time.perf_counter() - t_start:.0f} s)")
+    C1, C2 = 200, 20
+    prob = AdjointProblem(sysm, load, nsteps, dt, beta=beta, method="matfree", obs_steps=obs,
+                          u_obs=u_obs, checkpoint_every=C1, inner_every=C2)
+    prob.set_theta(torch.as_tensor(theta).cuda())
+    torch.cuda.reset_peak_memory_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = prob.forward()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"forward {t1 - t0:.1f} s")
+    g = prob.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log(f"adjoint {t2 - t1:.1f} s")
+    fwd, adj = t1 - t0, t2 - t1
+    peak = torch.cuda.max_memory_allocated() / 1e9
+    out = {
+        "metric": "forward + adjoint gradient on the stenosed 512k-node tube, 256 samples, 2,000 steps",
+        "value": S * nsteps / (fwd + adj), "unit": "sample-steps/s (forward + adjoint)", "n_gpus": 1,
+        "steps": nsteps, "warmup": 5, "ms_per_step": 1e3 * (fwd + adj) / nsteps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded stenosed tube and GMRF samples",
+        "config": {"workload": case.name, "n_nodes": N, "n_elements": E, "nnzb": sysm.nnzb,
+                   "n_samples": S, "dt": dt, "beta": beta, "observations": len(obs),
+                   "checkpoints": [C1, C2], "adjoint_operator": "matfree"},
+        "step_operators": per_step, "stored_vs_matfree_rel_max_err": agree,
+        "forward_s": fwd, "adjoint_s": adj, "adjoint_over_forward": adj / fwd,
+        "peak_device_GB": peak, "loss_mean": float(loss.mean()),
         "gradient_max_abs": float(g.abs().max()),
     }
     print(json.dumps(out), flush=True)

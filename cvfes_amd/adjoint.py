@@ -8,8 +8,13 @@ steps, evaluates the per-sample loss
 and runs the reverse sweep: for each segment between checkpoints the forward states are recomputed
 from the checkpoint, then ``cvfes_adjoint_step`` (the transposed step) and ``cvfes_grad_accum``
 (the element-wise derivative of the stiffness) run backwards through the segment.  Memory is one
-pair of states per checkpoint plus one segment of states.  ``EnsembleLoss`` wraps it as a
-``torch.autograd.Function`` of ``theta = log E [E, S]``.
+pair of states per checkpoint plus one segment of states.  With ``inner_every = C2`` the sweep is
+two-level: a segment of ``C`` steps is first recomputed keeping a pair every ``C2`` steps, and
+only ``C2`` states are held at a time, so memory falls from ``2 n/C + C`` to
+``2 n/C + 2 C/C2 + C2`` states for one more forward pass.  Observations may live in host memory
+(copied to the device when used).  Recomputation replays the same arithmetic, so the gradient does
+not depend on the checkpoint scheme.  ``EnsembleLoss`` wraps it as a ``torch.autograd.Function`` of
+``theta = log E [E, S]``.
 
 All device work is in the native kernels; torch provides the buffers, the per-sample loss
 reduction and the autograd plumbing.
@@ -36,12 +41,15 @@ class AdjointProblem:
     load : :class:`LoadTables` with at least ``nsteps`` entries (entry ``n`` = load of step ``n``).
     nsteps, dt, beta, damp, method : as for the forward step.
     obs_steps : sorted step indices ``k`` (1..nsteps) at which ``u_k`` is observed.
-    u_obs : device tensor ``[len(obs_steps), N, 3, S]`` of observations (``None`` = zero).
+    u_obs : tensor ``[len(obs_steps), N, 3, S]`` of observations (``None`` = zero), on the device or
+        in (preferably pinned) host memory.
     checkpoint_every : checkpoint interval ``C`` (1 = keep every state, no recomputation).
+    inner_every : second-level interval ``C2`` inside a segment (``None`` = one level).
     """
 
     def __init__(self, system: MembraneSystem, load: LoadTables, nsteps, dt, beta=0.0, damp=0.0,
-                 method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None):
+                 method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None,
+                 inner_every=None):
         self.sys = system
         self.load = load
         self.nsteps = int(nsteps)
@@ -53,6 +61,8 @@ class AdjointProblem:
         self.obs_index = {k: i for i, k in enumerate(self.obs_steps)}
         self.u_obs = u_obs
         self.C = max(1, int(checkpoint_every))
+        self.C2 = None if inner_every is None or int(inner_every) >= self.C else max(1, int(inner_every))
+        self._obs_dev = None
         N = system.n_nodes
         self.u0 = np.zeros(3 * N) if u0 is None else np.asarray(u0, np.float64)
         self.du0 = np.zeros(3 * N) if du0 is None else np.asarray(du0, np.float64)
@@ -64,7 +74,15 @@ class AdjointProblem:
         self.sys.set_log_modulus(theta)
 
     def _obs(self, k):
-        return None if self.u_obs is None else self.u_obs[self.obs_index[k]]
+        if self.u_obs is None:
+            return None
+        ref = self.u_obs[self.obs_index[k]]
+        if ref.device == self.sys.device:
+            return ref
+        if self._obs_dev is None:
+            self._obs_dev = torch.empty(ref.shape, dtype=torch.float64, device=self.sys.device)
+        self._obs_dev.copy_(ref, non_blocking=True)
+        return self._obs_dev
 
     def _loss_term(self, u, k):
         ref = self._obs(k)
@@ -134,31 +152,23 @@ class AdjointProblem:
         # mu_N = P g_N
         if N in self.obs_index:
             self._adjoint_call(zeros, zeros, mu[0], self.u_final, self._obs(N), w)
-        mu_n1, mu_n2, spare = mu[0], zeros.clone(), mu[1]    # mu_{n+1}, mu_{n+2}
-        seg_starts = sorted(self.checkpoints, reverse=True)
-        seg_bufs = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]
-        for s0 in seg_starts:
+        self._mu = [mu[0], zeros.clone(), mu[1]]              # mu_{n+1}, mu_{n+2}, spare
+        self._w, self._grad = w, grad
+        leaf = self.C if self.C2 is None else self.C2
+        self._seg_bufs = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(leaf - 1)]
+        self._tmp = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)] \
+            if self.C2 is not None else []
+        for s0 in sorted(self.checkpoints, reverse=True):
             s1 = min(s0 + self.C, N)
-            # recompute u_{s0+1} .. u_{s1-1} from the checkpoint (u_{s0}, u_{s0-1}), writing each new
-            # state straight into its segment buffer
-            u_s0, u_sm1 = self.checkpoints[s0]
-            states = {s0 - 1: u_sm1, s0: u_s0}
-            for i, n in enumerate(range(s0, s1 - 1)):
-                sysm.state = [states[n], states[n - 1], seg_bufs[i]]
-                self._advance(n, 1)
-                states[n + 1] = seg_bufs[i]
-            for n in range(s1 - 1, s0 - 1, -1):
-                # gradient of step n: mu_{n+1}, x_n = u_n + (beta/dt)(u_n - u_{n-1})
-                self._grad_call(mu_n1, states[n], states[n - 1] if self.beta != 0.0 else None,
-                                self.dt * self.dt, grad)
-                if n == 0:
-                    break
-                obs_u = states[n] if n in self.obs_index else None
-                self._adjoint_call(mu_n1, mu_n2, spare, obs_u,
-                                   self._obs(n) if obs_u is not None else None,
-                                   w if obs_u is not None else None)
-                mu_n2, mu_n1, spare = mu_n1, spare, mu_n2
-            del states
+            if self.C2 is None:
+                self._reverse_leaf(s0, s1, self.checkpoints[s0])
+            else:
+                pairs = self._recompute_pairs(s0, s1, self.checkpoints[s0], self.C2)
+                for t0 in sorted(pairs, reverse=True):
+                    self._reverse_leaf(t0, min(t0 + self.C2, s1), pairs[t0])
+                del pairs
+        mu_n1 = self._mu[0]
+        self._seg_bufs = self._tmp = None
         sysm.state = None
         # initial state: u_{-1} depends on K through (u0 + beta du0)
         x0 = self.u0 + self.beta * self.du0
@@ -168,6 +178,45 @@ class AdjointProblem:
             x0_d = torch.as_tensor(np.repeat(x0.reshape(-1, 3)[:, :, None], S, axis=2)).to(dev)
             self._grad_call(lam_m1, x0_d, None, 0.5 * self.dt * self.dt, grad)
         return grad
+
+    def _recompute_pairs(self, s0, s1, pair, C2):
+        """Forward from the pair (u_{s0}, u_{s0-1}) to step s1 - 1, keeping (u_k, u_{k-1}) for
+        k = s0 + j C2 < s1; returns {k: pair} including s0."""
+        sysm = self.sys
+        pairs = {s0: pair}
+        cur, prev = pair
+        for n in range(s0, s1 - 1):
+            work = next(b for b in self._tmp if b is not cur and b is not prev)
+            sysm.state = [cur, prev, work]
+            self._advance(n, 1)
+            prev, cur = cur, sysm.state[0]
+            if (n + 1 - s0) % C2 == 0:
+                pairs[n + 1] = (cur.clone(), prev.clone())
+        return pairs
+
+    def _reverse_leaf(self, s0, s1, pair):
+        """Recompute u_{s0+1} .. u_{s1-1} from (u_{s0}, u_{s0-1}), then run the adjoint and gradient
+        steps n = s1-1 .. s0."""
+        sysm = self.sys
+        u_s0, u_sm1 = pair
+        states = {s0 - 1: u_sm1, s0: u_s0}
+        for i, n in enumerate(range(s0, s1 - 1)):
+            sysm.state = [states[n], states[n - 1], self._seg_bufs[i]]
+            self._advance(n, 1)
+            states[n + 1] = self._seg_bufs[i]
+        mu_n1, mu_n2, spare = self._mu
+        for n in range(s1 - 1, s0 - 1, -1):
+            # gradient of step n: mu_{n+1}, x_n = u_n + (beta/dt)(u_n - u_{n-1})
+            self._grad_call(mu_n1, states[n], states[n - 1] if self.beta != 0.0 else None,
+                            self.dt * self.dt, self._grad)
+            if n == 0:
+                break
+            obs_u = states[n] if n in self.obs_index else None
+            self._adjoint_call(mu_n1, mu_n2, spare, obs_u,
+                               self._obs(n) if obs_u is not None else None,
+                               self._w if obs_u is not None else None)
+            mu_n2, mu_n1, spare = mu_n1, spare, mu_n2
+        self._mu = [mu_n1, mu_n2, spare]
 
 
 class EnsembleLoss(torch.autograd.Function):

@@ -171,6 +171,69 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
     finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
 }
 
+// Stored step on the symmetric storage: the row's blocks are read from the upper entries
+// (uvalues[vblk[p]]), transposed for the lower ones (col < row).  Every upper block is read twice per
+// step, once by its own row and once, transposed, by its column's row; the second read comes
+// shortly after the first in the XCD-ordered row sweep (rows +1, +ntheta later), so it is served by
+// the caches and the HBM value stream is about halved.  With NTL the second (last) read is
+// non-temporal.  Loads are issued for the whole row before the arithmetic, as in the row-load kernel.
+template <bool BETA, bool DAMP, bool NTL, int MAXB, int VW>
+__global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *__restrict__ uvalues,
+                                                     const int32_t *__restrict__ vblk, int64_t step) {
+    typedef typename VecT<VW>::type V;
+    int32_t row;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    const RowArgs &k = a.k;
+    const int64_t S = k.n_smp;
+    const int32_t p0 = k.indptr[row];
+    const int32_t nb = k.indptr[row + 1] - p0;
+    V uo[3], po[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int64_t off = (static_cast<int64_t>(row) * 3 + i) * S + s;
+        uo[i] = ldv<VW, false>(a.u + off);
+        po[i] = ldv<VW, false>(a.up + off);
+    }
+    V acc[3] = {splat<VW>(0.0), splat<VW>(0.0), splat<VW>(0.0)};
+    if (nb > 0) {
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const bool valid = b < nb;
+            const int32_t pb = p0 + (valid ? b : nb - 1);
+            const int32_t col = k.indices[pb];
+            const bool tr = col < row;
+            V w[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int64_t off = (static_cast<int64_t>(col) * 3 + j) * S + s;
+                const V xu = ldv<VW, false>(a.u + off);
+                if constexpr (BETA) {
+                    const V xp = ldv<VW, false>(a.up + off);
+                    w[j] = xu + a.bdt * (xu - xp);
+                } else {
+                    w[j] = xu;
+                }
+            }
+            const double *base = uvalues + static_cast<int64_t>(vblk[pb]) * 9 * S + s;
+            V kv[9];
+            if (NTL && tr) {
+#pragma unroll
+                for (int ij = 0; ij < 9; ++ij) kv[ij] = ldv<VW, true>(base + (3 * (ij % 3) + ij / 3) * S);
+            } else {
+#pragma unroll
+                for (int ij = 0; ij < 9; ++ij) kv[ij] = ldv<VW, false>(base + (tr ? 3 * (ij % 3) + ij / 3 : ij) * S);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[i] += sel<VW>(valid, kv[3 * i + j]) * w[j];
+        }
+    }
+    finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
+}
+
 // Matrix-free step with each row's element data staged in LDS (step_row_lds); one wavefront per
 // (row, 64*VW samples), so rows must be wave-uniform.
 template <bool BETA, bool DAMP, int VW, int MAXB, int MAXE>
@@ -185,6 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_step_lds(StepArgs a, LdsArgs l, int6
 
 // values[p][ij][s] = sum_c coef[celem[c]][s] * khat[c][ij]; one thread per (block, sample).
 __global__ __launch_bounds__(kBlock) void k_assemble(int64_t nnzb, int64_t n_smp,
+                                                      const int32_t *__restrict__ bmap,
                                                       const int32_t *__restrict__ cptr,
                                                       const int32_t *__restrict__ celem,
                                                       const double *__restrict__ khat,
@@ -192,12 +256,13 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t nnzb, int64_t n_smp
                                                       double *__restrict__ values) {
     const int64_t g = xcd_block() * kBlock + threadIdx.x;
     if (g >= nnzb * n_smp) return;
-    const int64_t p = g / n_smp;
+    const int64_t p = g / n_smp;                  // output block
     const int64_t s = g - p * n_smp;
+    const int64_t b = bmap ? bmap[p] : p;         // block of the full structure
     double kv[9];
 #pragma unroll
     for (int ij = 0; ij < 9; ++ij) kv[ij] = 0.0;
-    for (int32_t c = cptr[p]; c < cptr[p + 1]; ++c) {
+    for (int32_t c = cptr[b]; c < cptr[b + 1]; ++c) {
         const double cf = coef[static_cast<int64_t>(celem[c]) * n_smp + s];
         const double *kh = khat + 9 * static_cast<int64_t>(c);
 #pragma unroll
@@ -355,6 +420,22 @@ void launch_lds_bd(hipStream_t st, const StepArgs &a, const LdsArgs &l, int64_t 
     }
 }
 
+template <bool NTL, int MAXB>
+void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step, bool beta,
+                bool damp) {
+    const int64_t work = a.k.n_nodes * lanes_per_row<true, 2>(a.k.n_smp);
+    const dim3 grid(grid_for(work)), block(kBlock);
+    const double *uv = sys->uvalues;
+    const int32_t *vb = sys->vblk;
+    if (beta) {
+        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        else hipLaunchKernelGGL((k_step_sym<true, false, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+    } else {
+        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        else hipLaunchKernelGGL((k_step_sym<false, false, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+    }
+}
+
 // LDS slots needed by the structure: 1 = (MAXB 7, MAXE 6), 2 = (MAXB 10, MAXE 9), 0 = too large.
 int lds_slots(const cvfes_system_t *sys) {
     if (!sys->nptr || !sys->nent || !sys->cslot || sys->max_node_elems < 1 || sys->max_row_blocks < 1)
@@ -401,6 +482,19 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
         launch_lds(st, sys, a, step, beta, damp, two);
         return;
     }
+    if (!mf && (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT) && maxb && sys->uvalues && sys->vblk) {
+        const bool ntl = v == CVFES_VARIANT_SYM_NT;
+        if (maxb == 7) {
+            if (ntl) launch_sym<true, 7>(st, sys, a, step, beta, damp);
+            else launch_sym<false, 7>(st, sys, a, step, beta, damp);
+        } else {
+            if (ntl) launch_sym<true, 10>(st, sys, a, step, beta, damp);
+            else launch_sym<false, 10>(st, sys, a, step, beta, damp);
+        }
+        return;
+    }
+    if (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT)
+        v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
     // LDS staging only exists for the matrix-free operator (and structures within its limits)
     if (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2)
         v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
@@ -439,8 +533,20 @@ extern "C" int cvfes_assemble_bcsr(void *stream, int64_t nnzb, int64_t n_smp, co
     if (nnzb == 0) return CVFES_OK;
     CVFES_REQUIRE(cptr && celem && khat && coef && values, "cvfes_assemble_bcsr: null pointer");
     hipLaunchKernelGGL(k_assemble, dim3(grid_for(nnzb * n_smp)), dim3(kBlock), 0,
-                       static_cast<hipStream_t>(stream), nnzb, n_smp, cptr, celem, khat, coef,
-                       values);
+                       static_cast<hipStream_t>(stream), nnzb, n_smp, nullptr, cptr, celem, khat,
+                       coef, values);
+    return check_launch("k_assemble");
+}
+
+extern "C" int cvfes_assemble_upper(void *stream, int64_t nnzu, int64_t n_smp, const int32_t *ublk,
+                                    const int32_t *cptr, const int32_t *celem, const double *khat,
+                                    const double *coef, double *uvalues) {
+    CVFES_REQUIRE(nnzu >= 0 && n_smp >= 1, "cvfes_assemble_upper: bad sizes");
+    if (nnzu == 0) return CVFES_OK;
+    CVFES_REQUIRE(ublk && cptr && celem && khat && coef && uvalues, "cvfes_assemble_upper: null pointer");
+    hipLaunchKernelGGL(k_assemble, dim3(grid_for(nnzu * n_smp)), dim3(kBlock), 0,
+                       static_cast<hipStream_t>(stream), nnzu, n_smp, ublk, cptr, celem, khat,
+                       coef, uvalues);
     return check_launch("k_assemble");
 }
 
@@ -500,7 +606,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     CVFES_REQUIRE(nsteps >= 0, "cvfes_step: nsteps < 0");
     CVFES_REQUIRE(dt > 0.0, "cvfes_step: dt must be positive");
     CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_step: lm_sstride must be 0 or 1");
-    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_LDS2,
+    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_SYM_NT,
                   "cvfes_step: unknown variant %lld", static_cast<long long>(sys->variant));
     hipStream_t st = static_cast<hipStream_t>(stream);
     StepArgs a{};

@@ -12,7 +12,9 @@ is built with vectorised sorting instead of per-node Python lists, with an ident
   ``assembling`` (``optimizedSolidAssemble.pyx:133-188``) adds them into ``K[p]``;
 * ``cpos[e*9 + ab]``: the inverse map (where element ``e``'s block ``ab`` lands);
 * ``nptr``/``nent``: node -> incident (element, local index) lists, used to pull lumped mass and
-  nodal loads without scatter races.
+  nodal loads without scatter races;
+* ``ublk``/``vblk``: symmetric storage - the upper blocks (col >= row) in block order, and for every
+  block the upper entry that holds it (itself, or its mirror (col, row) read transposed).
 """
 from __future__ import annotations
 
@@ -97,5 +99,14 @@ class BlockMaps:
         where = np.searchsorted(qkeys[qorder], ckeys)
         self.cslot = slot_of_q[qorder][where].astype(np.int32)
         self.max_node_elems = int(np.diff(self.nptr).max()) if n_nodes else 0
+        # symmetric storage: upper blocks and, per block, the upper entry holding it
+        up = indices >= row_of_block
+        self.ublk = np.nonzero(up)[0].astype(np.int32)
+        uindex = np.cumsum(up) - 1                              # upper index of upper blocks
+        mirror = np.searchsorted(row_keys, indices * n_nodes + row_of_block)
+        if np.any(row_keys[np.minimum(mirror, nnzb - 1)] != indices * n_nodes + row_of_block):
+            raise ValueError("block structure is not symmetric")
+        self.vblk = np.where(up, uindex, uindex[mirror]).astype(np.int32)
+        self.nnzu = int(self.ublk.size)
         self.cptr = self.cptr.astype(np.int32)
         self.nptr = self.nptr.astype(np.int32)

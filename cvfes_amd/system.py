@@ -118,6 +118,10 @@ class MembraneSystem:
         self.nent = _dev(maps.nent, i32, dev)
         self.cslot = _dev(maps.cslot, i32, dev)
         self.max_node_elems = maps.max_node_elems
+        self.ublk = _dev(maps.ublk, i32, dev)
+        self.vblk = _dev(maps.vblk, i32, dev)
+        self.nnzu = maps.nnzu
+        self.uvalues = None
         fixed = np.zeros(self.n_nodes, np.uint8)
         fixed[np.asarray(boundary, np.int64)] = 1
         self.fixed = _dev(fixed, torch.uint8, dev)
@@ -201,12 +205,27 @@ class MembraneSystem:
             self.values = torch.empty((self.nnzb, 9, S), dtype=torch.float64, device=self.device)
         check(self.lib.cvfes_assemble_bcsr(self.stream, self.nnzb, S, ptr(self.cptr), ptr(self.celem),
                                            ptr(self.khat), ptr(self.coef), ptr(self.values)))
+        if self.uvalues is not None:
+            self.assemble_upper()
+        if hasattr(self, "_sys"):
+            self._sys = self._make_struct()
+
+    def assemble_upper(self):
+        """Form (or re-form) the symmetric storage ``uvalues [nnzu][9][S]`` (upper blocks only) used
+        by the ``sym`` step variants."""
+        S = self.n_smp
+        if self.uvalues is None:
+            self.uvalues = torch.empty((self.nnzu, 9, S), dtype=torch.float64, device=self.device)
+        check(self.lib.cvfes_assemble_upper(self.stream, self.nnzu, S, ptr(self.ublk), ptr(self.cptr),
+                                            ptr(self.celem), ptr(self.khat), ptr(self.coef),
+                                            ptr(self.uvalues)))
         if hasattr(self, "_sys"):
             self._sys = self._make_struct()
 
     def release_values(self):
         """Drop the stored values (matrix-free runs only need coef and khat)."""
         self.values = None
+        self.uvalues = None
         self._sys = self._make_struct()
 
     def _make_struct(self):
@@ -218,11 +237,16 @@ class MembraneSystem:
             bdy_u=self.bdy_u, max_row_blocks=self.max_row_blocks,
             max_offdiag_contrib=self.max_offdiag_contrib, max_diag_contrib=self.max_diag_contrib,
             variant=self.variant, nptr=ptr(self.nptr), nent=ptr(self.nent), cslot=ptr(self.cslot),
-            max_node_elems=self.max_node_elems)
+            max_node_elems=self.max_node_elems, uvalues=ptr(self.uvalues), vblk=ptr(self.vblk),
+            nnzu=self.nnzu)
 
     def set_variant(self, variant):
-        """Select the step kernel variant (``auto``, ``generic``, ``rowload``, ``rowload_nt``)."""
+        """Select the step kernel variant (names in ``_lib.VARIANTS``); the symmetric ones form the
+        upper-block storage on first use."""
         self.variant = _lib.VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        if self.variant in (_lib.VARIANT_SYM, _lib.VARIANT_SYM_NT) and self.uvalues is None \
+                and self.values is not None:
+            self.assemble_upper()
         self._sys = self._make_struct()
 
     @property

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 2
+#define CVFES_ABI_VERSION 3
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -67,6 +67,13 @@ typedef struct cvfes_system {
     const int32_t *nent;      /* [3E] e*3 + local index                                         */
     const int32_t *cslot;     /* [9E]                                                           */
     int64_t max_node_elems;   /* exact maximum of nptr[a+1] - nptr[a] (0 = unknown)             */
+    /* Symmetric storage of the stored operator (SYM variants; may be NULL otherwise): only the
+     * blocks (a, b) with b >= a are stored, uvalues[q][ij][s]; vblk[p] is the upper entry holding
+     * block p of the full structure, itself when col(p) >= row(p), else the entry of the
+     * mirrored block (col, row), read transposed. */
+    const double *uvalues;    /* [nnzu][9][S]                                                   */
+    const int32_t *vblk;      /* [nnzb]                                                         */
+    int64_t nnzu;
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */
@@ -78,6 +85,9 @@ typedef struct cvfes_system {
 #define CVFES_VARIANT_ROWLOAD1_NT 5 /* as ROWLOAD1 with non-temporal value loads                   */
 #define CVFES_VARIANT_LDS 6        /* matrix-free: row coefficients and element blocks staged in LDS */
 #define CVFES_VARIANT_LDS2 7       /* as LDS with two samples per lane                               */
+#define CVFES_VARIANT_SYM 8        /* stored, symmetric: upper blocks only, lower ones read
+                                      transposed from their mirror (about half the value bytes)   */
+#define CVFES_VARIANT_SYM_NT 9     /* as SYM, mirrored (second) reads non-temporal                 */
 
 /* Time-dependent nodal load: f_n = p[n] * f1 + qa[n] * ga + qb[n] * gb  (ga/gb may be NULL).
  * ga, gb are sample-independent [N][3] traction fields (e.g. two consecutive fluid wall-stress
@@ -93,6 +103,10 @@ typedef struct cvfes_load {
 
 /* ABI version compiled into the library (CVFES_ABI_VERSION). */
 int cvfes_abi_version(void);
+
+/* sizeof of the ABI structs as compiled (binding checks): 0 cvfes_system_t, 1 cvfes_load_t,
+ * 2 cvfes_batch_mesh_t, 3 cvfes_geometry_t, 4 cvfes_update_t; -1 otherwise.  Host only. */
+int64_t cvfes_struct_size(int which);
 
 /* Description of the last error raised on the calling thread ("" if none). */
 const char *cvfes_last_error(void);
@@ -134,6 +148,17 @@ int cvfes_lumped_mass_load(void *stream, int64_t n_nodes, int64_t n_smp, const i
                            const int32_t *nent, const int32_t *elems, const double *area,
                            const double *normal, const double *vthick, int64_t thick_sstride,
                            double density, double mass_scale, double *lm, double *f1);
+
+/*
+ * Symmetric stored values: uvalues[q][ij][s] = values[ublk[q]][ij][s] for the nnzu upper blocks
+ * (col >= row) listed by ublk.  K is symmetric (K[a][b] = K[b][a]^T), so the step reads block p of
+ * the full structure as uvalues[vblk[p]], transposed when col(p) < row(p).
+ * Replaces: the K half of assembling() (optimizedSolidAssemble.pyx:133-188) for the symmetric
+ *   storage of the same matrix.
+ */
+int cvfes_assemble_upper(void *stream, int64_t nnzu, int64_t n_smp, const int32_t *ublk,
+                         const int32_t *cptr, const int32_t *celem, const double *khat,
+                         const double *coef, double *uvalues);
 
 /*
  * Stored block-CSR values: values[p][ij][s] = sum_{c in block p} coef[celem[c], s] * khat[c][ij].

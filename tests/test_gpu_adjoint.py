@@ -3,8 +3,8 @@
 * small tube: the GPU adjoint equals torch autograd through the same recurrence (1e-9 relative),
   for both operators, with Rayleigh beta, damping, a non-zero initial displacement (gradient
   through u_{-1}) and a checkpoint interval that does not divide the step count;
-* the gradient does not depend on the checkpoint interval (bit-exact: recomputation replays the
-  same arithmetic);
+* the gradient does not depend on the checkpoint scheme (one or two levels, observations on the
+  device or in host memory; bit-exact: recomputation replays the same arithmetic);
 * 49,152-node tube, 32 samples, 2,000 steps, 20 observation times, checkpoints every 100 steps:
   central finite differences with step 1e-6 on 20 seeded (element, sample) pairs, each within
   1e-6 relative of the adjoint.  Two separate 2,000-step FP64 runs at theta +- h would carry a
@@ -87,13 +87,18 @@ def test_gradient_independent_of_checkpoint_interval(require_gpu):
     p = sine(nsteps, dt)
     obs = [30, 60, 90]
     u_obs = torch.zeros((3, nodes.shape[0], 3, 8), dtype=torch.float64, device="cuda")
+    u_obs[1] = 1e-4                       # a non-zero observation, also kept in host memory
+    u_host = u_obs.cpu().pin_memory()
     grads = []
-    for C in (1, 10, 90, 1000):
+    for C, C2, host in ((1, None, False), (10, None, False), (90, None, False), (1000, None, False),
+                        (30, 7, False), (45, 10, True), (90, 1, True)):
         prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, method="stored",
-                              obs_steps=obs, u_obs=u_obs, checkpoint_every=C)
+                              obs_steps=obs, u_obs=u_host if host else u_obs, checkpoint_every=C,
+                              inner_every=C2)
         prob.set_theta(torch.as_tensor(theta).cuda())
         prob.forward()
         grads.append(prob.backward().cpu().numpy())
+    assert np.max(np.abs(grads[0])) > 0
     for g in grads[1:]:
         assert np.array_equal(g, grads[0])
 

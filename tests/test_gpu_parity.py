@@ -145,7 +145,7 @@ def case128(require_gpu):
 
 
 @pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt",
-                                     "lds", "lds2"])
+                                     "lds", "lds2", "sym", "sym_nt"])
 @pytest.mark.parametrize("method", ["stored", "matfree"])
 def test_kernel_variants_match_oracle(case128, method, variant):
     """Every step-kernel variant at 128 samples (wave-uniform rows) against the oracle."""

@@ -203,6 +203,20 @@ def test_library_exports_every_declared_symbol():
     assert h.cvfes_abi_version() == _lib.ABI_VERSION
 
 
+def test_ctypes_mirrors_match_compiled_struct_sizes():
+    """Every ctypes mirror has the size of the C struct it stands for (a field added on one side
+    only would shift every later field of arrays such as the batch descriptors)."""
+    import ctypes
+    from cvfes_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libcvfes_mi355x.so not built")
+    lib = _lib.lib()
+    mirrors = [_lib.CvfesSystem, _lib.CvfesLoad, _lib.CvfesBatchMesh, _lib.CvfesGeometry, _lib.CvfesUpdate]
+    for which, cls in enumerate(mirrors):
+        assert lib.cvfes_struct_size(which) == ctypes.sizeof(cls), cls.__name__
+    assert lib.cvfes_struct_size(99) == -1
+
+
 def test_abi_argument_errors_without_gpu():
     """Argument validation runs on the host: null/invalid inputs return CVFES_ERR_ARG, no launch."""
     import ctypes

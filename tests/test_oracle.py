@@ -534,3 +534,21 @@ def test_stress_nodal_force_equals_element_stiffness_product():
     fl = h * area[:, None] * np.einsum("eki,ek->ei", B, sig)       # local nodal forces
     fs = np.einsum("eji,ej->ei", so._big_T(T), fl)
     assert np.abs(fs - fk).max() <= 1e-10 * np.abs(fk).max()
+
+
+def test_symmetric_block_maps_reproduce_every_block():
+    """Upper storage: block p equals uvalues[vblk[p]] (transposed when col < row)."""
+    nodes, el, bdy = tube_geometry(12, 6, 0.4, 1.5)
+    n = nodes.shape[0]
+    ip, ix = block_structure(el, n)
+    m = BlockMaps(el, n, ip, ix)
+    rng = np.random.default_rng(0)
+    coef = rng.uniform(1, 2, (el.shape[0], 2))
+    _, K, _, _ = so.assemble_reference(nodes, el, 1.0, so.d_matrix(0.3), coef, np.ones((el.shape[0], 2, 3)),
+                                       1.0, ip, ix)
+    row = np.repeat(np.arange(n), np.diff(ip))
+    assert m.nnzu == (ix.size + n) // 2 and np.all(ix[m.ublk] >= row[m.ublk])
+    Ku = K[m.ublk]
+    got = np.where((ix < row)[:, None, None, None], np.swapaxes(Ku[m.vblk], -1, -2), Ku[m.vblk])
+    assert np.array_equal(got[ix >= row], K[ix >= row])
+    assert np.abs(got - K).max() <= 1e-13 * np.abs(K).max()
