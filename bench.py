@@ -47,7 +47,9 @@ def algorithmic_bytes(sysm, method):
     state = 3 * (3 * N * S * 8)                 # read u, read up, write u+
     lm = sysm.lm.numel() * 8
     small = 3 * N * 8 + 4 * (N + 1) + 4 * nnzb + N   # f1, indptr, indices, fixed
-    if method == "stored":
+    if method == "stored" and getattr(sysm, "variant", 0) in (8, 9, 10):  # symmetric storage
+        op = sysm.nnzu * 9 * S * 8 + 4 * nnzb                          # upper values, vblk
+    elif method == "stored":
         op = nnzb * 9 * S * 8
     else:
         op = E * S * 8 + 9 * E * 9 * 8 + 4 * 9 * E + 4 * (nnzb + 1)   # coef, khat, celem, cptr
@@ -133,6 +135,7 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--case", default="tube-256x400-s128")
     ap.add_argument("--method", default="stored", choices=["stored", "matfree"])
+    ap.add_argument("--variant", default="auto", help="step kernel variant (cvfes_amd._lib.VARIANTS)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="CPU baseline steps (first is warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
     ap.add_argument("--no-cpu", action="store_true")
@@ -165,10 +168,12 @@ def main():
     dt, beta = case.dt, case.beta
     t_setup = time.perf_counter()
     nodes, el, bdy = case.geometry()
-    gm = case.sample_E(el)
+    from cvfes_amd.gmrf import sample_element_E
+    gm = sample_element_E(el, case.n_smp, seed=case.seed, E0=case.E0, cov=case.cov, device="cuda")
     N, S = nodes.shape[0], case.n_smp
     sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=gm.E,
                           density=case.density, nu=case.nu, store_values=True)
+    sysm.set_variant(args.variant)
     total = args.warmup + args.steps
     load = sysm.loads(case.pressures(0, max(total, args.cpu_steps + 1), dt))
     torch.cuda.synchronize()
@@ -191,25 +196,35 @@ def main():
         "data": "synthetic: seeded tube mesh and per-element GMRF Young's modulus samples",
         "config": {"workload": case.name, "n_nodes": N, "n_elements": int(el.shape[0]),
                    "nnzb": sysm.nnzb, "n_samples": S, "dt": dt, "beta": beta,
-                   "operator": args.method, "layout": "values [nnzb][9][S], state [N][3][S]"},
+                   "operator": args.method, "variant": args.variant,
+                   "layout": "values [nnzb][9][S], state [N][3][S]"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(case.name, args.method),
+                     "traffic": measured_traffic(case.name, args.method if args.variant == "auto"
+                                                 else f"{args.method}-{args.variant}"),
                      "algorithmic_bytes_per_launch": abytes, "kernel_ms": kern_ms,
                      "kernel": "k_step (fused block-CSR product + central-difference update)"},
     }
     if not args.no_matfree:
-        other = "matfree" if args.method == "stored" else "stored"
-        msteps = max(20, args.steps // 2)
-        mwall, mk = time_steps(sysm, load, dt, beta, other, args.warmup, msteps)
-        mbytes = algorithmic_bytes(sysm, other)
-        out["alternative"] = {"operator": other, "value": S * msteps / mwall,
-                              "unit": "sample-steps/s", "kernel_ms": mk,
-                              "achieved_GBs": mbytes / (mk * 1e-3) / 1e9,
-                              "frac": mbytes / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                              "algorithmic_bytes_per_launch": mbytes}
-        print(f"[bench] {other}: {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
-              file=sys.stderr, flush=True)
+        # the other operator and the symmetric storage of the stored one, measured the same way
+        alts = [("matfree", "auto")] if args.method == "stored" else [("stored", "auto")]
+        if args.method == "stored" and args.variant == "auto":
+            alts.append(("stored", "sym_ntl"))
+        out["alternatives"] = []
+        for other, variant in alts:
+            sysm.set_variant(variant)
+            msteps = max(20, args.steps // 2)
+            mwall, mk = time_steps(sysm, load, dt, beta, other, args.warmup, msteps)
+            mbytes = algorithmic_bytes(sysm, other)
+            key = other if variant == "auto" else f"{other}-{variant}"
+            out["alternatives"].append({
+                "operator": other, "variant": variant, "value": S * msteps / mwall,
+                "unit": "sample-steps/s", "kernel_ms": mk, "achieved_GBs": mbytes / (mk * 1e-3) / 1e9,
+                "frac": mbytes / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": mbytes,
+                "traffic": measured_traffic(case.name, key)})
+            print(f"[bench] {key}: {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
+                  file=sys.stderr, flush=True)
+        sysm.set_variant(args.variant)
     if not args.no_cpu:
         from oracle import cpu_oracle as co
         threads = args.cpu_threads or co.max_threads()

@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -23,11 +23,11 @@ METHODS = {"stored": METHOD_STORED, "bcsr": METHOD_STORED, "matfree": METHOD_MAT
 VARIANT_AUTO, VARIANT_GENERIC, VARIANT_ROWLOAD, VARIANT_ROWLOAD_NT = 0, 1, 2, 3
 VARIANT_ROWLOAD1, VARIANT_ROWLOAD1_NT = 4, 5
 VARIANT_LDS, VARIANT_LDS2 = 6, 7
-VARIANT_SYM, VARIANT_SYM_NT = 8, 9
+VARIANT_SYM, VARIANT_SYM_NT, VARIANT_SYM_NTL = 8, 9, 10
 VARIANTS = {"auto": VARIANT_AUTO, "generic": VARIANT_GENERIC, "rowload": VARIANT_ROWLOAD,
             "rowload_nt": VARIANT_ROWLOAD_NT, "rowload1": VARIANT_ROWLOAD1,
             "rowload1_nt": VARIANT_ROWLOAD1_NT, "lds": VARIANT_LDS, "lds2": VARIANT_LDS2,
-            "sym": VARIANT_SYM, "sym_nt": VARIANT_SYM_NT}
+            "sym": VARIANT_SYM, "sym_nt": VARIANT_SYM_NT, "sym_ntl": VARIANT_SYM_NTL}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -49,7 +49,7 @@ class CvfesSystem(ctypes.Structure):
         ("lm", _vp), ("lm_sstride", _i64), ("f1", _vp), ("fixed", _vp), ("bdy_u", _f64),
         ("max_row_blocks", _i64), ("max_offdiag_contrib", _i64), ("max_diag_contrib", _i64),
         ("variant", _i64), ("nptr", _vp), ("nent", _vp), ("cslot", _vp), ("max_node_elems", _i64),
-        ("uvalues", _vp), ("vblk", _vp), ("nnzu", _i64),
+        ("uvalues", _vp), ("vblk", _vp), ("nnzu", _i64), ("row_order", _vp),
     ]
 
 

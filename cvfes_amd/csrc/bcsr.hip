@@ -42,6 +42,9 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a, int64_t step) {
     int64_t s;
     bool active;
     if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    if constexpr (UNIFORM) {
+        if (a.order) row = a.order[row];
+    }
     step_row<MATFREE, BETA, DAMP, VW>(a, row, s, active, step);
 }
 
@@ -66,6 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
     int64_t s;
     bool active;
     if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    if (a.order) row = a.order[row];
     const RowArgs &k = a.k;
     const int64_t S = k.n_smp;
     const int32_t p0 = k.indptr[row];
@@ -173,11 +177,12 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
 
 // Stored step on the symmetric storage: the row's blocks are read from the upper entries
 // (uvalues[vblk[p]]), transposed for the lower ones (col < row).  Every upper block is read twice per
-// step, once by its own row and once, transposed, by its column's row; the second read comes
-// shortly after the first in the XCD-ordered row sweep (rows +1, +ntheta later), so it is served by
-// the caches and the HBM value stream is about halved.  With NTL the second (last) read is
-// non-temporal.  Loads are issued for the whole row before the arithmetic, as in the row-load kernel.
-template <bool BETA, bool DAMP, bool NTL, int MAXB, int VW>
+// step, once by its own row and once, transposed, by its column's row, 1 and ntheta rows later in
+// the XCD-ordered sweep, so the second read is served by the caches and the HBM value stream is
+// about halved.  Value loads: NT = 0 plain, 1 all non-temporal, 2 non-temporal for the mirrored
+// (second) reads only.  Loads are issued for the whole row before the arithmetic, as in the
+// row-load kernel.
+template <bool BETA, bool DAMP, int NT, int MAXB, int VW>
 __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *__restrict__ uvalues,
                                                      const int32_t *__restrict__ vblk, int64_t step) {
     typedef typename VecT<VW>::type V;
@@ -185,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
     int64_t s;
     bool active;
     if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    if (a.order) row = a.order[row];
     const RowArgs &k = a.k;
     const int64_t S = k.n_smp;
     const int32_t p0 = k.indptr[row];
@@ -218,12 +224,13 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
             }
             const double *base = uvalues + static_cast<int64_t>(vblk[pb]) * 9 * S + s;
             V kv[9];
-            if (NTL && tr) {
+            if (NT == 2 && tr) {
 #pragma unroll
                 for (int ij = 0; ij < 9; ++ij) kv[ij] = ldv<VW, true>(base + (3 * (ij % 3) + ij / 3) * S);
             } else {
 #pragma unroll
-                for (int ij = 0; ij < 9; ++ij) kv[ij] = ldv<VW, false>(base + (tr ? 3 * (ij % 3) + ij / 3 : ij) * S);
+                for (int ij = 0; ij < 9; ++ij)
+                    kv[ij] = ldv<VW, NT == 1>(base + (tr ? 3 * (ij % 3) + ij / 3 : ij) * S);
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -420,7 +427,7 @@ void launch_lds_bd(hipStream_t st, const StepArgs &a, const LdsArgs &l, int64_t 
     }
 }
 
-template <bool NTL, int MAXB>
+template <int NT, int MAXB>
 void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step, bool beta,
                 bool damp) {
     const int64_t work = a.k.n_nodes * lanes_per_row<true, 2>(a.k.n_smp);
@@ -428,11 +435,11 @@ void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, in
     const double *uv = sys->uvalues;
     const int32_t *vb = sys->vblk;
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
-        else hipLaunchKernelGGL((k_step_sym<true, false, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        else hipLaunchKernelGGL((k_step_sym<true, false, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
     } else {
-        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
-        else hipLaunchKernelGGL((k_step_sym<false, false, NTL, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        else hipLaunchKernelGGL((k_step_sym<false, false, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
     }
 }
 
@@ -482,19 +489,21 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
         launch_lds(st, sys, a, step, beta, damp, two);
         return;
     }
-    if (!mf && (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT) && maxb && sys->uvalues && sys->vblk) {
-        const bool ntl = v == CVFES_VARIANT_SYM_NT;
+    const bool sym = v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT || v == CVFES_VARIANT_SYM_NTL;
+    if (!mf && sym && maxb && sys->uvalues && sys->vblk) {
+        const int nt = v == CVFES_VARIANT_SYM ? 0 : (v == CVFES_VARIANT_SYM_NT ? 1 : 2);
         if (maxb == 7) {
-            if (ntl) launch_sym<true, 7>(st, sys, a, step, beta, damp);
-            else launch_sym<false, 7>(st, sys, a, step, beta, damp);
+            if (nt == 0) launch_sym<0, 7>(st, sys, a, step, beta, damp);
+            else if (nt == 1) launch_sym<1, 7>(st, sys, a, step, beta, damp);
+            else launch_sym<2, 7>(st, sys, a, step, beta, damp);
         } else {
-            if (ntl) launch_sym<true, 10>(st, sys, a, step, beta, damp);
-            else launch_sym<false, 10>(st, sys, a, step, beta, damp);
+            if (nt == 0) launch_sym<0, 10>(st, sys, a, step, beta, damp);
+            else if (nt == 1) launch_sym<1, 10>(st, sys, a, step, beta, damp);
+            else launch_sym<2, 10>(st, sys, a, step, beta, damp);
         }
         return;
     }
-    if (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT)
-        v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
+    if (sym) v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
     // LDS staging only exists for the matrix-free operator (and structures within its limits)
     if (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2)
         v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
@@ -606,7 +615,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     CVFES_REQUIRE(nsteps >= 0, "cvfes_step: nsteps < 0");
     CVFES_REQUIRE(dt > 0.0, "cvfes_step: dt must be positive");
     CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_step: lm_sstride must be 0 or 1");
-    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_SYM_NT,
+    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_SYM_NTL,
                   "cvfes_step: unknown variant %lld", static_cast<long long>(sys->variant));
     hipStream_t st = static_cast<hipStream_t>(stream);
     StepArgs a{};
@@ -620,6 +629,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     a.dt2 = dt * dt;
     a.bdt = beta / dt;
     a.ddt = damp / dt;
+    a.order = sys->row_order;
     const bool mf = method == CVFES_METHOD_MATFREE;
     double *cur = state[0], *prev = state[1], *nxt = state[2];
     for (int64_t n = 0; n < nsteps; ++n) {

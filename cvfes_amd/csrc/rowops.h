@@ -64,6 +64,7 @@ struct StepArgs {
     double dt2;
     double bdt;           // beta / dt
     double ddt;           // damp / dt
+    const int32_t *order; // visiting order of the rows (wave-uniform kernels; null = natural)
 };
 
 template <bool MATFREE, int VW>

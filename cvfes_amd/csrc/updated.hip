@@ -253,6 +253,7 @@ extern "C" int cvfes_step_updated(void *stream, const cvfes_system_t *sys, const
         a.dt2 = dt * dt;
         a.bdt = beta / dt;
         a.ddt = damp / dt;
+        a.order = nullptr;
         const GeomArgs g = geom_args(geo, sys, geom);
         if (uni) launch_updated<true>(st, a, g, n, beta != 0.0, damp != 0.0);
         else launch_updated<false>(st, a, g, n, beta != 0.0, damp != 0.0);

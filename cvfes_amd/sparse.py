@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["SparseInfo", "BlockMaps", "block_structure"]
+__all__ = ["SparseInfo", "BlockMaps", "block_structure", "bisection_order"]
 
 
 def block_structure(elements: np.ndarray, n_nodes: int):
@@ -34,6 +34,27 @@ def block_structure(elements: np.ndarray, n_nodes: int):
     indptr = np.zeros(n_nodes + 1, dtype=np.int64)
     np.cumsum(np.bincount(row_of, minlength=n_nodes), out=indptr[1:])
     return indptr, indices
+
+
+def bisection_order(nodes: np.ndarray, leaf: int = 64) -> np.ndarray:
+    """Compact visiting order of the nodes: recursive coordinate bisection (split the longest extent
+    at the median) down to ``leaf`` nodes, parts in tree order.  Neighbouring nodes end up a few
+    positions apart, so a block read by one row is read again by its mirror row soon after."""
+    nodes = np.asarray(nodes, np.float64)
+    out = []
+    stack = [np.arange(nodes.shape[0])]
+    while stack:
+        idx = stack.pop()
+        if idx.size <= leaf:
+            out.append(idx)
+            continue
+        pts = nodes[idx]
+        ax = int(np.argmax(pts.max(0) - pts.min(0)))
+        o = idx[np.argsort(pts[:, ax], kind="stable")]
+        h = o.size // 2
+        stack.append(o[h:])
+        stack.append(o[:h])
+    return np.concatenate(out).astype(np.int32) if out else np.zeros(0, np.int32)
 
 
 class SparseInfo:

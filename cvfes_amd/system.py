@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from ._lib import CvfesGeometry, CvfesLoad, CvfesSystem, CvfesUpdate, check, ptr
-from .sparse import BlockMaps, block_structure
+from .sparse import BlockMaps, bisection_order, block_structure
 
 __all__ = ["MembraneSystem", "LoadTables", "MASS_SCALE"]
 
@@ -122,6 +122,8 @@ class MembraneSystem:
         self.vblk = _dev(maps.vblk, i32, dev)
         self.nnzu = maps.nnzu
         self.uvalues = None
+        self._nodes_h = nodes
+        self.row_order = None
         fixed = np.zeros(self.n_nodes, np.uint8)
         fixed[np.asarray(boundary, np.int64)] = 1
         self.fixed = _dev(fixed, torch.uint8, dev)
@@ -238,15 +240,28 @@ class MembraneSystem:
             max_offdiag_contrib=self.max_offdiag_contrib, max_diag_contrib=self.max_diag_contrib,
             variant=self.variant, nptr=ptr(self.nptr), nent=ptr(self.nent), cslot=ptr(self.cslot),
             max_node_elems=self.max_node_elems, uvalues=ptr(self.uvalues), vblk=ptr(self.vblk),
-            nnzu=self.nnzu)
+            nnzu=self.nnzu, row_order=ptr(self.row_order))
 
     def set_variant(self, variant):
         """Select the step kernel variant (names in ``_lib.VARIANTS``); the symmetric ones form the
         upper-block storage on first use."""
         self.variant = _lib.VARIANTS[variant] if isinstance(variant, str) else int(variant)
-        if self.variant in (_lib.VARIANT_SYM, _lib.VARIANT_SYM_NT) and self.uvalues is None \
+        if self.variant in (_lib.VARIANT_SYM, _lib.VARIANT_SYM_NT, _lib.VARIANT_SYM_NTL) and self.uvalues is None \
                 and self.values is not None:
             self.assemble_upper()
+        self._sys = self._make_struct()
+
+    def set_row_order(self, order="bisection", leaf=64):
+        """Visiting order of the block rows for the wave-uniform stored kernels: ``"natural"``,
+        ``"bisection"`` (recursive coordinate bisection, parts of ``leaf`` nodes) or an explicit
+        permutation.  The result does not depend on it."""
+        if order is None or (isinstance(order, str) and order == "natural"):
+            self.row_order = None
+        else:
+            perm = bisection_order(self._nodes_h, leaf) if isinstance(order, str) else np.asarray(order)
+            if perm.shape != (self.n_nodes,) or not np.array_equal(np.sort(perm), np.arange(self.n_nodes)):
+                raise ValueError("row order must be a permutation of the nodes")
+            self.row_order = _dev(perm.astype(np.int32), torch.int32, self.device)
         self._sys = self._make_struct()
 
     @property

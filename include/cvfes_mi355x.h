@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 3
+#define CVFES_ABI_VERSION 4
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -74,6 +74,10 @@ typedef struct cvfes_system {
     const double *uvalues;    /* [nnzu][9][S]                                                   */
     const int32_t *vblk;      /* [nnzb]                                                         */
     int64_t nnzu;
+    /* Order in which the wave-uniform step kernels visit the block rows (a permutation of
+     * 0..N-1; NULL = natural).  Any permutation gives the same result; a compact one (neighbouring
+     * nodes close together) lets the second read of a mirrored block hit in L2. */
+    const int32_t *row_order;
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */
@@ -87,7 +91,8 @@ typedef struct cvfes_system {
 #define CVFES_VARIANT_LDS2 7       /* as LDS with two samples per lane                               */
 #define CVFES_VARIANT_SYM 8        /* stored, symmetric: upper blocks only, lower ones read
                                       transposed from their mirror (about half the value bytes)   */
-#define CVFES_VARIANT_SYM_NT 9     /* as SYM, mirrored (second) reads non-temporal                 */
+#define CVFES_VARIANT_SYM_NT 9     /* as SYM, every value load non-temporal                         */
+#define CVFES_VARIANT_SYM_NTL 10   /* as SYM, the mirrored (second) reads non-temporal              */
 
 /* Time-dependent nodal load: f_n = p[n] * f1 + qa[n] * ga + qb[n] * gb  (ga/gb may be NULL).
  * ga, gb are sample-independent [N][3] traction fields (e.g. two consecutive fluid wall-stress

@@ -145,7 +145,7 @@ def case128(require_gpu):
 
 
 @pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt",
-                                     "lds", "lds2", "sym", "sym_nt"])
+                                     "lds", "lds2", "sym", "sym_nt", "sym_ntl"])
 @pytest.mark.parametrize("method", ["stored", "matfree"])
 def test_kernel_variants_match_oracle(case128, method, variant):
     """Every step-kernel variant at 128 samples (wave-uniform rows) against the oracle."""
@@ -176,3 +176,24 @@ def test_padded_wavefront_rows_match_oracle(require_gpu, method, n_smp):
     y = c.sys.spmv(torch.as_tensor(x).cuda(), method).cpu().numpy()
     yref = so.multiply_by_vector(c.indptr, c.indices, c.K, x.reshape(-1, c.S))
     assert rel_err(y.reshape(-1, c.S), yref) < 1e-13
+
+
+@pytest.mark.parametrize("method,variant", [("stored", "rowload_nt"), ("stored", "sym"),
+                                            ("stored", "sym_nt"), ("stored", "sym_ntl"),
+                                            ("stored", "generic"), ("matfree", "generic")])
+def test_row_order_does_not_change_result(case128, method, variant):
+    """Visiting the rows in a bisection order or a random order gives bit-identical states."""
+    s = case128.sys
+    outs = []
+    perm = np.random.default_rng(0).permutation(s.n_nodes)
+    for order in ("natural", "bisection", perm):
+        s.set_variant(variant)
+        s.set_row_order(order)
+        try:
+            out, snaps = _run_both(case128, 40, method, beta=1e-6, damp=30.0, dt=3e-6, record=(40,))
+        finally:
+            s.set_row_order("natural")
+            s.set_variant("auto")
+        outs.append(out[40])
+        assert rel_err(out[40], snaps[40]) < REL_U
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
