@@ -283,7 +283,7 @@ def run_batch(args):
         dt = 0.8 * s.critical_dt(iters=30, method="matfree")[0]
         systems.append(s)
         dts.append(dt)
-        loads.append(s.loads(c.pressures(0, 2 * steps + 2 * ref_steps + 200, dt)))
+        loads.append(s.loads(c.pressures(0, 3 * steps + 6 * ref_steps + 600, dt)))
         if i % 16 == 0:
             print(f"[bench] batch mesh {i}: N={c.n_nodes} S={c.n_smp} ({time.perf_counter() - t0:.0f} s)",
                   file=sys.stderr, flush=True)
