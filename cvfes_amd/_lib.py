@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 4
+ABI_VERSION = 6
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -24,10 +24,12 @@ VARIANT_AUTO, VARIANT_GENERIC, VARIANT_ROWLOAD, VARIANT_ROWLOAD_NT = 0, 1, 2, 3
 VARIANT_ROWLOAD1, VARIANT_ROWLOAD1_NT = 4, 5
 VARIANT_LDS, VARIANT_LDS2 = 6, 7
 VARIANT_SYM, VARIANT_SYM_NT, VARIANT_SYM_NTL = 8, 9, 10
+VARIANT_CACHED, VARIANT_CACHED1 = 11, 12
 VARIANTS = {"auto": VARIANT_AUTO, "generic": VARIANT_GENERIC, "rowload": VARIANT_ROWLOAD,
             "rowload_nt": VARIANT_ROWLOAD_NT, "rowload1": VARIANT_ROWLOAD1,
             "rowload1_nt": VARIANT_ROWLOAD1_NT, "lds": VARIANT_LDS, "lds2": VARIANT_LDS2,
-            "sym": VARIANT_SYM, "sym_nt": VARIANT_SYM_NT, "sym_ntl": VARIANT_SYM_NTL}
+            "sym": VARIANT_SYM, "sym_nt": VARIANT_SYM_NT, "sym_ntl": VARIANT_SYM_NTL,
+            "cached": VARIANT_CACHED, "cached1": VARIANT_CACHED1}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -76,7 +78,7 @@ class CvfesBatchMesh(ctypes.Structure):
 
     _fields_ = [("sys", CvfesSystem), ("method", _i64), ("state", _vp * 3), ("load", CvfesLoad),
                 ("dt", _f64), ("beta", _f64), ("damp", _f64), ("task0", _i64), ("lanes", _i64),
-                ("vw", _i64), ("packed", _i64)]
+                ("vw", _i64), ("packed", _i64), ("n_mtasks", _i64), ("slice", _i64), ("xtask0", _i64)]
 
 
 _SIGNATURES = {
@@ -105,10 +107,10 @@ _SIGNATURES = {
                                   ctypes.POINTER(CvfesUpdate), ctypes.POINTER(_vp),
                                   ctypes.POINTER(CvfesLoad), _i64, _f64, _f64, _f64]),
     "cvfes_batch_plan": (_int, [_vp, ctypes.POINTER(CvfesBatchMesh), _int, _vp,
-                                ctypes.POINTER(_i64)]),
-    "cvfes_batch_step": (_int, [_vp, _vp, _int, _i64, _int, _vp, _i64, _i64]),
+                                ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "cvfes_batch_step": (_int, [_vp, _vp, _int, _i64, _int, _vp, _i64, _i64, _i64]),
     "cvfes_counter_add": (_int, [_vp, _vp, _i64]),
-    "cvfes_batch_graph_create": (_int, [_vp, _int, _i64, _vp, _i64, _int, _i64,
+    "cvfes_batch_graph_create": (_int, [_vp, _int, _i64, _vp, _i64, _int, _i64, _i64,
                                         ctypes.POINTER(_vp)]),
     "cvfes_graph_launch": (_int, [_vp, _vp]),
     "cvfes_graph_destroy": (_int, [_vp]),

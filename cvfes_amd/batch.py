@@ -7,8 +7,9 @@ count, operator, time step, damping and load tables), numbers their wave tasks o
 * ``run(n, graph=True)`` replays HIP graphs that hold ``block`` batched steps (a multiple of 3, so
   the rotating state roles come back to the start) and a device step counter increment; the
   remainder runs as eager launches;
-* ``autotune()`` times candidate persistent grid sizes (from a few wavefronts per SIMD up to one
-  wavefront per task) and keeps the fastest;
+* ``autotune()`` times candidate lane widths (two samples per lane where the count is even, or
+  one everywhere: fewer registers, more waves in flight) and persistent grid sizes (from a few
+  wavefronts per SIMD up to one wavefront per task) and keeps the fastest;
 * ``run_per_mesh(n, method)`` is the comparison path: each mesh's own single-mesh step kernel
   launched per mesh per step (with its own operator, or the one given).
 """
@@ -42,6 +43,8 @@ class BatchedEnsemble:
         self.device = torch.device(device) if device is not None else self.systems[0].device
         self.lib = _lib.lib()
         self.n_waves = 0
+        self.vw_hint = 0          # 0: two samples per lane for even counts, 1: one everywhere
+        self.kinds = 0
         self.phase = 0
         self.step_index = 0
         self._graphs = {}
@@ -73,13 +76,15 @@ class BatchedEnsemble:
             m.state[0], m.state[1], m.state[2] = u.data_ptr(), work.data_ptr(), up.data_ptr()
             m.load = self.loads[i].struct(0)
             m.dt, m.beta, m.damp = self.dts[i], self.betas[i], self.damps[i]
+            m.vw = self.vw_hint
         self._bufs = [(s.state[0], s.state[2], s.state[1]) for s in self.systems]   # state[0..2]
         self.dev_meshes = torch.empty(ctypes.sizeof(arr), dtype=torch.uint8, device=self.device)
-        n_tasks = ctypes.c_int64(0)
+        n_tasks, kinds = ctypes.c_int64(0), ctypes.c_int64(0)
         check(self.lib.cvfes_batch_plan(self.stream, arr, n, self.dev_meshes.data_ptr(),
-                                        ctypes.byref(n_tasks)))
+                                        ctypes.byref(n_tasks), ctypes.byref(kinds)))
         self.host_meshes = arr
         self.n_tasks = int(n_tasks.value)
+        self.kinds = int(kinds.value)
         self._drop_graphs()
 
     def _drop_graphs(self):
@@ -98,17 +103,18 @@ class BatchedEnsemble:
         for _ in range(n):
             check(self.lib.cvfes_batch_step(self.stream, self.dev_meshes.data_ptr(),
                                             len(self.systems), self.n_tasks, self.phase, None,
-                                            self.step_index, self.n_waves))
+                                            self.step_index, self.n_waves, self.kinds))
             self.phase = (self.phase + 1) % 3
             self.step_index += 1
 
     def _graph(self, block):
-        key = (block, self.phase, self.n_waves)
+        key = (block, self.phase, self.n_waves, self.kinds)
         if key not in self._graphs:
             g = ctypes.c_void_p()
             check(self.lib.cvfes_batch_graph_create(self.dev_meshes.data_ptr(), len(self.systems),
                                                     self.n_tasks, self.counter.data_ptr(), block,
-                                                    self.phase, self.n_waves, ctypes.byref(g)))
+                                                    self.phase, self.n_waves, self.kinds,
+                                                    ctypes.byref(g)))
             self._graphs[key] = g.value
         return self._graphs[key]
 
@@ -143,25 +149,38 @@ class BatchedEnsemble:
         self.phase = 0
 
     # ------------------------------------------------------------------------------------------
-    def autotune(self, candidates=None, steps=6):
-        """Pick the persistent grid size (wavefronts) with the shortest batched step."""
-        if candidates is None:
-            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-            candidates = [cus * w for w in (8, 16, 32, 64, 128)] + [self.n_tasks]
+    def autotune(self, candidates=None, steps=6, lane_widths=(0, 1)):
+        """Pick the lane width and persistent grid size (wavefronts) with the shortest step."""
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         best = None
         saved = (self.phase, self.step_index)
-        for w in candidates:
-            self.n_waves = w
-            self._eager(1)
-            torch.cuda.synchronize(self.device)
-            t0 = time.perf_counter()
-            self._eager(steps)
-            torch.cuda.synchronize(self.device)
-            dt = (time.perf_counter() - t0) / steps
-            if best is None or dt < best[1]:
-                best = (w, dt)
-        self.n_waves = best[0]
+        for vw in lane_widths:
+            self.vw_hint = vw
+            self._sync_roles()
+            self._replan()
+            grid = candidates if candidates is not None else \
+                [cus * w for w in (8, 16, 32, 64, 128)] + [self.n_tasks]
+            for w in grid:
+                self.n_waves = w
+                self._eager(1)
+                torch.cuda.synchronize(self.device)
+                t0 = time.perf_counter()
+                self._eager(steps)
+                torch.cuda.synchronize(self.device)
+                dt = (time.perf_counter() - t0) / steps
+                if best is None or dt < best[2]:
+                    best = (vw, w, dt)
+        self.vw_hint, self.n_waves = best[0], best[1]
         # the tuning steps advanced the state: restart the bookkeeping from the current roles
         self._sync_roles()
-        self.tuned = {"n_waves": best[0], "ms_per_step": best[1] * 1e3, "from_step": saved[1]}
+        self._replan()
+        self.tuned = {"lane_width": "one sample per lane" if best[0] == 1 else "two where even",
+                      "n_waves": best[1], "ms_per_step": best[2] * 1e3, "from_step": saved[1]}
         return self.tuned
+
+    def _replan(self):
+        """Re-plan from the meshes' current buffer roles (keeps the step index)."""
+        step = self.step_index
+        self._plan()
+        self.phase = 0
+        self.step_index = step

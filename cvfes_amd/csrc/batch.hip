@@ -6,9 +6,10 @@
 // whole wavefronts gets wave-uniform rows (each row padded to whole wavefronts: the mesh
 // descriptor, row pointers and column indices come in through scalar loads); otherwise its rows
 // are packed back to back so that no lane idles (row data through vector loads), as the
-// single-mesh kernels do.  Tasks are numbered mesh by mesh, so a contiguous task range is a band
-// of rows of one or two meshes.  Each wavefront of the persistent grid takes one contiguous range;
-// wave order is XCD-contiguous, so neighbouring rows (which share gathered columns) run on one XCD
+// single-mesh kernels do.  Every mesh's tasks are cut into 8 contiguous slices, one per XCD
+// (workgroup b runs on XCD b mod 8); an XCD's waves walk its slice of every mesh, mesh after mesh,
+// each wave a contiguous range.  So the XCDs get equal shares of every mesh whatever the meshes'
+// sizes and per-task costs, and neighbouring rows (which share gathered columns) run on one XCD
 // and its L2.  The row arithmetic is the single-mesh step_row (rowops.h).
 //
 // Buffer roles rotate with `phase` (0, 1, 2): u = state[phase], u_prev = state[(phase+2)%3],
@@ -68,34 +69,40 @@ __device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes,
-                                                        int n_mesh, int64_t n_tasks, int phase,
-                                                        const int64_t *__restrict__ counter,
-                                                        int64_t k_local) {
+template <bool HAS_VW2, bool HAS_VW1>
+__device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh,
+                                           int64_t n_tasks, int phase,
+                                           const int64_t *__restrict__ counter, int64_t k_local) {
     constexpr int64_t kWavesPerBlock = kBlock / 64;
-    const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
-    const int64_t wave = __builtin_amdgcn_readfirstlane(
-        static_cast<int32_t>(xcd_block() * kWavesPerBlock + threadIdx.x / 64));
+    // The grid is a multiple of 8 blocks and block b runs on XCD b mod 8: the XCD's waves walk the
+    // XCD's item list (its slice of every mesh, mesh after mesh) in contiguous ranges.
+    const int64_t xcd = blockIdx.x % kNumXcd;
+    const int64_t waves_x = static_cast<int64_t>(gridDim.x / kNumXcd) * kWavesPerBlock;
+    const int64_t lw = __builtin_amdgcn_readfirstlane(
+        static_cast<int32_t>((blockIdx.x / kNumXcd) * kWavesPerBlock + threadIdx.x / 64));
     const int lane = threadIdx.x & 63;
-    const int64_t per = (n_tasks + waves - 1) / waves;
-    const int64_t t0 = wave * per;
-    const int64_t t1 = t0 + per < n_tasks ? t0 + per : n_tasks;
-    if (t0 >= t1) return;
+    const int64_t items = n_tasks / kNumXcd;
+    const int64_t per = (items + waves_x - 1) / waves_x;
+    const int64_t i0 = lw * per;
+    const int64_t i1 = i0 + per < items ? i0 + per : items;
+    if (i0 >= i1) return;
     const int64_t n = (counter ? counter[0] : 0) + k_local;
-    // Mesh of the first task: meshes are numbered in task order, so it is the number of meshes with
-    // task0 <= t0, minus one.  Each lane reads one descriptor's task0 (all loads of the wave issue
-    // together, 64 meshes per round) and a ballot counts them.
+    // Mesh of the first item: the number of meshes with xtask0 <= i0, minus one.  Each lane reads
+    // one descriptor's xtask0 (all loads of the wave issue together, 64 meshes per round) and a
+    // ballot counts them.
     int mi = -1;
     for (int base = 0; base < n_mesh; base += 64) {
         const int m = base + lane;
-        const bool le = m < n_mesh && meshes[m].task0 <= t0;
+        const bool le = m < n_mesh && meshes[m].xtask0 <= i0;
         mi += __popcll(__ballot(le));
     }
-    for (int64_t t = t0; t < t1; ++t) {
-        while (mi + 1 < n_mesh && t >= meshes[mi + 1].task0) ++mi;
+    for (int64_t i = i0; i < i1; ++i) {
+        while (mi + 1 < n_mesh && i >= meshes[mi + 1].xtask0) ++mi;
         const cvfes_batch_mesh_t &d = meshes[mi];
+        const int64_t local = xcd * d.slice + (i - d.xtask0);    // task within the mesh
+        if (local >= d.n_mtasks) continue;                       // the last slice is short
         // lane index inside the mesh; 32-bit (cvfes_batch_plan checks n_nodes * lanes < 2^31)
-        const uint32_t g = static_cast<uint32_t>((t - d.task0) * 64) + lane;
+        const uint32_t g = static_cast<uint32_t>(local * 64) + lane;
         const uint32_t lanes = static_cast<uint32_t>(d.lanes);
         const int64_t S = d.sys.n_smp;
         const int vw = static_cast<int>(d.vw);
@@ -106,17 +113,42 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const cvfes_batch_mesh_t 
             const bool active = in && s < S;
             const int32_t row = in ? static_cast<int32_t>(r) : static_cast<int32_t>(d.sys.n_nodes - 1);
             if (!active) s = S - vw;
-            if (vw == 2) batch_dispatch<2>(d, phase, n, row, s, active);
-            else batch_dispatch<1>(d, phase, n, row, s, active);
+            if (HAS_VW2 && (!HAS_VW1 || vw == 2)) batch_dispatch<2>(d, phase, n, row, s, active);
+            else if (HAS_VW1) batch_dispatch<1>(d, phase, n, row, s, active);
         } else {
             const int32_t row = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(g / lanes));
             int64_t s = static_cast<int64_t>(g - static_cast<uint32_t>(row) * lanes) * vw;
             const bool active = s < S;
             if (!active) s = S - vw;
-            if (vw == 2) batch_dispatch<2>(d, phase, n, row, s, active);
-            else batch_dispatch<1>(d, phase, n, row, s, active);
+            if (HAS_VW2 && (!HAS_VW1 || vw == 2)) batch_dispatch<2>(d, phase, n, row, s, active);
+            else if (HAS_VW1) batch_dispatch<1>(d, phase, n, row, s, active);
         }
     }
+}
+
+template <bool HAS_VW2, bool HAS_VW1>
+__global__ __launch_bounds__(kBlock) void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes,
+                                                        int n_mesh, int64_t n_tasks, int phase,
+                                                        const int64_t *__restrict__ counter,
+                                                        int64_t k_local) {
+    batch_body<HAS_VW2, HAS_VW1>(meshes, n_mesh, n_tasks, phase, counter, k_local);
+}
+
+// One sample per lane everywhere: capped at 72 registers (7 waves per SIMD, no spills), against
+// 80 registers (6 waves) uncapped.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7)))
+void k_batch_step_vw1(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh, int64_t n_tasks,
+                      int phase, const int64_t *__restrict__ counter, int64_t k_local) {
+    batch_body<false, true>(meshes, n_mesh, n_tasks, phase, counter, k_local);
+}
+
+using BatchKernel = void (*)(const cvfes_batch_mesh_t *, int, int64_t, int, const int64_t *, int64_t);
+
+BatchKernel batch_kernel(int64_t kinds) {
+    const bool v2 = kinds & 1, v1 = kinds & 2;
+    if (v2 && v1) return k_batch_step<true, true>;
+    if (v2) return k_batch_step<true, false>;
+    return k_batch_step_vw1;
 }
 
 __global__ void k_counter_add(int64_t *counter, int64_t by) {
@@ -145,9 +177,10 @@ int default_waves() {
 using namespace cvfes;
 
 extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh,
-                                void *dev_meshes, int64_t *n_tasks) {
-    CVFES_REQUIRE(meshes && n_mesh >= 1 && dev_meshes && n_tasks, "cvfes_batch_plan: bad arguments");
-    int64_t t = 0;
+                                void *dev_meshes, int64_t *n_tasks, int64_t *kinds) {
+    CVFES_REQUIRE(meshes && n_mesh >= 1 && dev_meshes && n_tasks && kinds,
+                  "cvfes_batch_plan: bad arguments");
+    int64_t t = 0, k = 0, x = 0;
     for (int i = 0; i < n_mesh; ++i) {
         cvfes_batch_mesh_t &m = meshes[i];
         const cvfes_system_t &s = m.sys;
@@ -164,16 +197,23 @@ extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_
                       "cvfes_batch_plan: mesh %d load tables", i);
         CVFES_REQUIRE(m.dt > 0.0, "cvfes_batch_plan: mesh %d dt", i);
         CVFES_REQUIRE(s.lm_sstride == 0 || s.lm_sstride == 1, "cvfes_batch_plan: mesh %d lm_sstride", i);
-        m.vw = (s.n_smp % 2 == 0) ? 2 : 1;
+        CVFES_REQUIRE(m.vw == 0 || m.vw == 1 || m.vw == 2, "cvfes_batch_plan: mesh %d vw hint", i);
+        m.vw = (m.vw != 1 && s.n_smp % 2 == 0) ? 2 : 1;
+        k |= m.vw == 2 ? 1 : 2;
         const int64_t l = s.n_smp / m.vw, padded = (l + 63) / 64 * 64;
         m.packed = 4 * l >= 3 * padded ? 0 : 1;
         m.lanes = m.packed ? l : padded;
         CVFES_REQUIRE(s.n_nodes * m.lanes < (int64_t(1) << 31) - 64,
                       "cvfes_batch_plan: mesh %d has too many row lanes", i);
         m.task0 = t;
-        t += (s.n_nodes * m.lanes + 63) / 64;
+        m.n_mtasks = (s.n_nodes * m.lanes + 63) / 64;
+        m.slice = (m.n_mtasks + kNumXcd - 1) / kNumXcd;
+        m.xtask0 = x;
+        t += m.n_mtasks;
+        x += m.slice;
     }
-    *n_tasks = t;
+    *n_tasks = x * kNumXcd;
+    *kinds = k;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t err = hipMemcpyAsync(dev_meshes, meshes, sizeof(cvfes_batch_mesh_t) * n_mesh,
                                     hipMemcpyHostToDevice, st);
@@ -184,12 +224,13 @@ extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_
 
 extern "C" int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks,
                                 int phase, const int64_t *counter, int64_t k_local,
-                                int64_t n_waves) {
+                                int64_t n_waves, int64_t kinds) {
     CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && n_tasks >= 0, "cvfes_batch_step: bad arguments");
     CVFES_REQUIRE(phase >= 0 && phase < 3, "cvfes_batch_step: phase must be 0, 1 or 2");
+    CVFES_REQUIRE(kinds >= 1 && kinds <= 3, "cvfes_batch_step: kinds must come from cvfes_batch_plan");
     if (n_tasks == 0) return CVFES_OK;
     const int64_t w = n_waves > 0 ? n_waves : default_waves();
-    hipLaunchKernelGGL(k_batch_step, dim3(waves_to_blocks(w)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(batch_kernel(kinds), dim3(waves_to_blocks(w)), dim3(kBlock), 0,
                        static_cast<hipStream_t>(stream),
                        static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks, phase,
                        counter, k_local);
@@ -205,7 +246,8 @@ extern "C" int cvfes_counter_add(void *stream, int64_t *counter, int64_t by) {
 
 extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int64_t n_tasks,
                                         int64_t *counter, int64_t block_steps, int phase0,
-                                        int64_t n_waves, void **graph) {
+                                        int64_t n_waves, int64_t kinds, void **graph) {
+    CVFES_REQUIRE(kinds >= 1 && kinds <= 3, "cvfes_batch_graph_create: kinds must come from cvfes_batch_plan");
     CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && counter && graph, "cvfes_batch_graph_create: bad arguments");
     CVFES_REQUIRE(block_steps >= 3 && block_steps % 3 == 0,
                   "cvfes_batch_graph_create: block_steps must be a positive multiple of 3");
@@ -220,7 +262,7 @@ extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int6
         return fail(CVFES_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(err));
     }
     for (int64_t k = 0; k < block_steps; ++k)
-        hipLaunchKernelGGL(k_batch_step, dim3(waves_to_blocks(w)), dim3(kBlock), 0, cs,
+        hipLaunchKernelGGL(batch_kernel(kinds), dim3(waves_to_blocks(w)), dim3(kBlock), 0, cs,
                            static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks,
                            static_cast<int>((phase0 + k) % 3), counter, k);
     hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, cs, counter, block_steps);

@@ -241,6 +241,18 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
     finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
 }
 
+// Matrix-free step with the row's coefficients loaded once per row (step_row_cached); one
+// wavefront per (row, 64*VW samples), so rows must be wave-uniform.
+template <bool BETA, bool DAMP, int VW, int MAXE>
+__global__ __launch_bounds__(kBlock) void k_step_cached(StepArgs a, LdsArgs l, int64_t step) {
+    int32_t row;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    if (a.order) row = a.order[row];
+    step_row_cached<BETA, DAMP, VW, MAXE>(a, l, row, s, active, step);
+}
+
 // Matrix-free step with each row's element data staged in LDS (step_row_lds); one wavefront per
 // (row, 64*VW samples), so rows must be wave-uniform.
 template <bool BETA, bool DAMP, int VW, int MAXB, int MAXE>
@@ -443,6 +455,20 @@ void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, in
     }
 }
 
+template <int VW, int MAXE>
+void launch_cached(hipStream_t st, const StepArgs &a, const LdsArgs &l, int64_t step, bool beta,
+                   bool damp) {
+    const int64_t work = a.k.n_nodes * lanes_per_row<true, VW>(a.k.n_smp);
+    const dim3 grid(grid_for(work)), block(kBlock);
+    if (beta) {
+        if (damp) hipLaunchKernelGGL((k_step_cached<true, true, VW, MAXE>), grid, block, 0, st, a, l, step);
+        else hipLaunchKernelGGL((k_step_cached<true, false, VW, MAXE>), grid, block, 0, st, a, l, step);
+    } else {
+        if (damp) hipLaunchKernelGGL((k_step_cached<false, true, VW, MAXE>), grid, block, 0, st, a, l, step);
+        else hipLaunchKernelGGL((k_step_cached<false, false, VW, MAXE>), grid, block, 0, st, a, l, step);
+    }
+}
+
 // LDS slots needed by the structure: 1 = (MAXB 7, MAXE 6), 2 = (MAXB 10, MAXE 9), 0 = too large.
 int lds_slots(const cvfes_system_t *sys) {
     if (!sys->nptr || !sys->nent || !sys->cslot || sys->max_node_elems < 1 || sys->max_row_blocks < 1)
@@ -489,6 +515,21 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
         launch_lds(st, sys, a, step, beta, damp, two);
         return;
     }
+    if (mf && (v == CVFES_VARIANT_CACHED || v == CVFES_VARIANT_CACHED1) && lds_slots(sys) &&
+        use_uniform<2>(a.k.n_smp) && a.k.n_smp % 2 == 0) {
+        const LdsArgs l{sys->nptr, sys->nent, sys->cslot};
+        const bool six = lds_slots(sys) == 1;
+        if (v == CVFES_VARIANT_CACHED) {
+            if (six) launch_cached<2, 6>(st, a, l, step, beta, damp);
+            else launch_cached<2, 9>(st, a, l, step, beta, damp);
+        } else {
+            if (six) launch_cached<1, 6>(st, a, l, step, beta, damp);
+            else launch_cached<1, 9>(st, a, l, step, beta, damp);
+        }
+        return;
+    }
+    if (v == CVFES_VARIANT_CACHED || v == CVFES_VARIANT_CACHED1)
+        v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
     const bool sym = v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT || v == CVFES_VARIANT_SYM_NTL;
     if (!mf && sym && maxb && sys->uvalues && sys->vblk) {
         const int nt = v == CVFES_VARIANT_SYM ? 0 : (v == CVFES_VARIANT_SYM_NT ? 1 : 2);
@@ -615,7 +656,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     CVFES_REQUIRE(nsteps >= 0, "cvfes_step: nsteps < 0");
     CVFES_REQUIRE(dt > 0.0, "cvfes_step: dt must be positive");
     CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_step: lm_sstride must be 0 or 1");
-    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_SYM_NTL,
+    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_CACHED1,
                   "cvfes_step: unknown variant %lld", static_cast<long long>(sys->variant));
     hipStream_t st = static_cast<hipStream_t>(stream);
     StepArgs a{};

@@ -160,10 +160,11 @@ __device__ __forceinline__ void step_row(const StepArgs &a, int32_t row, int64_t
         Vec<VW> o;
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
+            // explicit fused operations: the same rounding for every lane width
             const double m = a.lm_sstride ? lmp[v] : lmp[0];
             double force = fi;
-            if constexpr (DAMP) force -= a.ddt * (uu.v[v] - pp.v[v]);
-            const double rhs = a.dt2 * (force - acc[i][v]) + m * (2.0 * uu.v[v] - pp.v[v]);
+            if constexpr (DAMP) force = fma(-a.ddt, uu.v[v] - pp.v[v], force);
+            const double rhs = fma(a.dt2, force - acc[i][v], m * fma(2.0, uu.v[v], -pp.v[v]));
             double un = m != 0.0 ? rhs / m : 0.0;
             o.v[v] = fixed ? a.bdy_u : un;
         }
@@ -360,6 +361,77 @@ __device__ __forceinline__ void step_row_lds(const StepArgs &a, const LdsArgs &l
                     for (int j = 0; j < 3; ++j) acc[i] += kv[3 * i + j] * w[b][j];
             }
         }
+    }
+    finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Matrix-free row step with the row node's element coefficients loaded once per row.
+//
+// In the generic matrix-free row every element contribution loads its coefficient again (a row
+// has 3 contributions per incident element) and each load sits in a dependent chain.  Here the
+// wavefront (one row, wave-uniform) first loads the coefficients of the node's incident elements
+// (at most MAXE, all issued together), then walks the blocks; each contribution picks its
+// element's coefficient from registers by its slot (cslot, uniform) and its unit block from the
+// scalar cache, so no vector load remains inside the contribution loop.
+template <bool BETA, bool DAMP, int VW, int MAXE>
+__device__ __forceinline__ void step_row_cached(const StepArgs &a, const LdsArgs &l, int32_t row,
+                                                int64_t s, bool active, int64_t step) {
+    typedef typename VecT<VW>::type V;
+    const RowArgs &k = a.k;
+    const int64_t S = k.n_smp;
+    const int32_t q0 = l.nptr[row];
+    const int32_t ne = l.nptr[row + 1] - q0;
+    V cf[MAXE];
+#pragma unroll
+    for (int e = 0; e < MAXE; ++e) cf[e] = splat<VW>(0.0);
+    if (ne > 0) {                   // an isolated node has no elements and no blocks
+#pragma unroll
+        for (int e = 0; e < MAXE; ++e) {
+            const int64_t el = l.nent[q0 + (e < ne ? e : ne - 1)] / 3;
+            cf[e] = ldv<VW, false>(k.coef + el * S + s);
+        }
+    }
+    V uo[3], po[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int64_t off = (static_cast<int64_t>(row) * 3 + i) * S + s;
+        uo[i] = ldv<VW, false>(a.u + off);
+        po[i] = ldv<VW, false>(a.up + off);
+    }
+    V acc[3] = {splat<VW>(0.0), splat<VW>(0.0), splat<VW>(0.0)};
+    const int32_t p0 = k.indptr[row], p1 = k.indptr[row + 1];
+    for (int32_t p = p0; p < p1; ++p) {
+        const int64_t col = k.indices[p];
+        V x[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int64_t off = (col * 3 + j) * S + s;
+            const V xu = ldv<VW, false>(a.u + off);
+            if constexpr (BETA) {
+                const V xp = ldv<VW, false>(a.up + off);
+                x[j] = xu + a.bdt * (xu - xp);
+            } else {
+                x[j] = xu;
+            }
+        }
+        V kv[9];
+#pragma unroll
+        for (int ij = 0; ij < 9; ++ij) kv[ij] = splat<VW>(0.0);
+        for (int32_t c = k.cptr[p]; c < k.cptr[p + 1]; ++c) {
+            const int32_t slot = l.cslot[c];
+            V cfc = cf[0];
+#pragma unroll
+            for (int e = 1; e < MAXE; ++e)
+                if (slot == e) cfc = cf[e];
+            const double *kh = k.khat + 9 * static_cast<int64_t>(c);
+#pragma unroll
+            for (int ij = 0; ij < 9; ++ij) kv[ij] += cfc * kh[ij];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) acc[i] += kv[3 * i + j] * x[j];
     }
     finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
 }

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 4
+#define CVFES_ABI_VERSION 6
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -93,6 +93,9 @@ typedef struct cvfes_system {
                                       transposed from their mirror (about half the value bytes)   */
 #define CVFES_VARIANT_SYM_NT 9     /* as SYM, every value load non-temporal                         */
 #define CVFES_VARIANT_SYM_NTL 10   /* as SYM, the mirrored (second) reads non-temporal              */
+#define CVFES_VARIANT_CACHED 11    /* matrix-free: the row node's element coefficients loaded once
+                                      per row (two samples per lane)                             */
+#define CVFES_VARIANT_CACHED1 12   /* as CACHED with one sample per lane                            */
 
 /* Time-dependent nodal load: f_n = p[n] * f1 + qa[n] * ga + qb[n] * gb  (ga/gb may be NULL).
  * ga, gb are sample-independent [N][3] traction fields (e.g. two consecutive fluid wall-stress
@@ -293,25 +296,35 @@ typedef struct cvfes_batch_mesh {
     double dt, beta, damp;
     int64_t task0;         /* set by cvfes_batch_plan: first wave task of the mesh             */
     int64_t lanes;         /* set by cvfes_batch_plan: lanes per block row                     */
-    int64_t vw;            /* set by cvfes_batch_plan: samples per lane (2 for even counts)    */
+    int64_t vw;            /* samples per lane: 0 on input = choose (2 for even counts, else 1),
+                              1 = one sample per lane; set by cvfes_batch_plan                 */
     int64_t packed;        /* set by cvfes_batch_plan: 1 = rows packed back to back in lanes,
                               0 = each row padded to whole wavefronts                         */
+    int64_t n_mtasks;      /* set by cvfes_batch_plan: wave tasks of the mesh                   */
+    int64_t slice;         /* set by cvfes_batch_plan: tasks per XCD slice, ceil(n_mtasks / 8)  */
+    int64_t xtask0;        /* set by cvfes_batch_plan: first item of the mesh in every XCD's list */
 } cvfes_batch_mesh_t;
 
 /* Number the wave tasks of `meshes` (host array, updated in place) and copy the array to
  * `dev_meshes` (device, n_mesh * sizeof(cvfes_batch_mesh_t) bytes).  A wave task is 64 lanes of
  * one mesh, each lane vw consecutive samples of one block row.  A mesh whose sample count fills at
  * least 3/4 of whole wavefronts gets wave-uniform rows (lanes = S/vw rounded up to 64: row data
- * through scalar loads); otherwise its rows are packed back to back (lanes = S/vw).  Blocks until
- * the copy is done. */
+ * through scalar loads); otherwise its rows are packed back to back (lanes = S/vw).  Every mesh is
+ * cut into 8 contiguous slices, one per XCD, and each XCD walks its slice of every mesh in turn, so
+ * the XCDs get equal shares of every mesh (balanced) and contiguous rows (neighbour states shared in
+ * their L2).  n_tasks receives 8 x the slice items of all meshes (the size of the task space; one
+ * wavefront per task is n_waves = n_tasks).  kinds
+ * receives which lane widths occur (bit 0: vw 2, bit 1: vw 1); pass it to cvfes_batch_step and
+ * cvfes_batch_graph_create, which then launch a kernel instance with only those paths (fewer
+ * registers, more waves in flight).  Blocks until the copy is done. */
 int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void *dev_meshes,
-                     int64_t *n_tasks);
+                     int64_t *n_tasks, int64_t *kinds);
 
 /* One step of every mesh (one launch).  Step index for the load tables: counter[0] + k_local
  * (k_local alone when counter is NULL).  n_waves: persistent grid size in wavefronts (0 = 16 per
  * compute unit). */
 int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks, int phase,
-                     const int64_t *counter, int64_t k_local, int64_t n_waves);
+                     const int64_t *counter, int64_t k_local, int64_t n_waves, int64_t kinds);
 
 /* counter[0] += by on the stream (a kernel, so that it can sit inside a captured graph). */
 int cvfes_counter_add(void *stream, int64_t *counter, int64_t by);
@@ -319,7 +332,8 @@ int cvfes_counter_add(void *stream, int64_t *counter, int64_t by);
 /* Capture block_steps (a multiple of 3) batched steps starting at phase0, followed by
  * counter += block_steps, into an executable HIP graph returned in *graph. */
 int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int64_t n_tasks, int64_t *counter,
-                             int64_t block_steps, int phase0, int64_t n_waves, void **graph);
+                             int64_t block_steps, int phase0, int64_t n_waves, int64_t kinds,
+                             void **graph);
 int cvfes_graph_launch(void *graph, void *stream);
 int cvfes_graph_destroy(void *graph);
 

@@ -2,7 +2,8 @@
 
 * a mixed ensemble (sample counts 5, 16, 33, 100, 128, 7; both operators; per-mesh dt, beta and
   damp) stepped in one launch per step equals each mesh stepped alone with the generic
-  single-mesh kernel, bit for bit;
+  single-mesh kernel, bit for bit, at both lane widths;
+* the autotuner (lane widths x grid sizes) leaves the trajectory unchanged;
 * graph replays (with eager work queued between the replays) equal eager batched steps, bit for
   bit, over several replays;
 * one batched mesh matches the CPU oracle (1e-10).
@@ -45,14 +46,19 @@ def states(systems):
     return [s.u.cpu().numpy().copy() for s in systems]
 
 
-def test_batched_equals_per_mesh(require_gpu):
+@pytest.mark.parametrize("vw_hint", [0, 1])
+def test_batched_equals_per_mesh(require_gpu, vw_hint):
+    """Both lane widths (two samples per lane where even / one everywhere) give the per-mesh
+    result bit for bit."""
     from cvfes_amd.batch import BatchedEnsemble
     nsteps = 40
     systems, loads = make_ensemble(nsteps)
     cols = list(zip(*SPECS))
     ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
                           betas=list(cols[6]), damps=list(cols[7]))
+    ens.vw_hint = vw_hint
     ens.initial_state()
+    assert ens.kinds == (3 if vw_hint == 0 else 2)
     ens.run(nsteps, graph=False)
     batched = states(systems)
     ens.initial_state()
@@ -130,3 +136,23 @@ def test_gmrf_device_solves_match_host(require_gpu):
     dev = sample_element_E(el, 8, seed=5, device="cuda")
     assert abs(dev.tau - host.tau) <= 1e-10 * host.tau
     assert np.max(np.abs(dev.z - host.z)) <= 1e-9 * np.max(np.abs(host.z))
+
+
+def test_autotune_keeps_the_trajectory(require_gpu):
+    from cvfes_amd.batch import BatchedEnsemble
+    nsteps = 3 * 11 + 40
+    systems, loads = make_ensemble(nsteps, seed=3)
+    cols = list(zip(*SPECS))
+    ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
+                          betas=list(cols[6]), damps=list(cols[7]))
+    ens.initial_state()
+    tuned = ens.autotune(candidates=[256, 2048], steps=2)
+    assert tuned["n_waves"] in (256, 2048)
+    done = ens.step_index
+    ens.run(nsteps - done, graph=True, block=33)
+    tuned_states = states(systems)
+    ens.vw_hint, ens.n_waves = 0, 0
+    ens.initial_state()
+    ens.run(nsteps, graph=False)
+    for a, b in zip(tuned_states, states(systems)):
+        assert np.array_equal(a, b)

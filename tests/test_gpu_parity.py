@@ -145,7 +145,8 @@ def case128(require_gpu):
 
 
 @pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt",
-                                     "lds", "lds2", "sym", "sym_nt", "sym_ntl"])
+                                     "lds", "lds2", "sym", "sym_nt", "sym_ntl",
+                                     "cached", "cached1"])
 @pytest.mark.parametrize("method", ["stored", "matfree"])
 def test_kernel_variants_match_oracle(case128, method, variant):
     """Every step-kernel variant at 128 samples (wave-uniform rows) against the oracle."""
@@ -197,3 +198,36 @@ def test_row_order_does_not_change_result(case128, method, variant):
         outs.append(out[40])
         assert rel_err(out[40], snaps[40]) < REL_U
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("method,variant", [("stored", "generic"), ("stored", "rowload_nt"),
+                                            ("stored", "sym_ntl"), ("matfree", "generic"),
+                                            ("matfree", "lds"), ("matfree", "cached"),
+                                            ("matfree", "cached1")])
+def test_isolated_node_every_kernel(require_gpu, method, variant):
+    """A node in no element (empty block row, zero mass) next to a tube: every kernel keeps it at
+    zero and matches the oracle elsewhere."""
+    import torch  # noqa: F401
+    from cvfes_amd.mesh import tube_geometry
+    from cvfes_amd.system import MembraneSystem
+    nodes, el, bdy = tube_geometry(24, 8, 0.3, 2.0)
+    nodes = np.vstack([nodes, [[5.0, 5.0, 5.0]]])            # isolated last node
+    N, S = nodes.shape[0], 128
+    rng = np.random.default_rng(4)
+    E = 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
+    sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, 0.07), elmE=E, density=1.0, nu=0.5)
+    sysm.set_variant(variant)
+    nsteps, dt = 60, 3e-6
+    p = 1.33e4 * (1 + 0.1 * np.sin(np.arange(nsteps)))
+    z = np.zeros(3 * N)
+    sysm.initial_state(z, z, sysm.loads(p), dt, beta=1e-6, damp=10.0, method=method)
+    sysm.step(p, dt, beta=1e-6, damp=10.0, method=method)
+    ug = sysm.u.cpu().numpy().reshape(-1, S)
+    ip, ix = so.sparse_info_loops(el, N)
+    gth = np.full((el.shape[0], S, 3), 0.07)
+    _, K, f1, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(0.5), so.coef_k_element_E(gth, E),
+                                         gth, 1.0, ip, ix)
+    bd = (3 * bdy[:, None] + np.arange(3)).ravel()
+    u, _, _ = so.run(ip, ix, K, LM, f1, p, z, z, dt, beta=1e-6, damp=10.0, bdy_dofs=bd)
+    assert np.all(ug[-3:] == 0.0)
+    assert rel_err(ug, u) < REL_U
