@@ -7,9 +7,11 @@ count, operator, time step, damping and load tables), numbers their wave tasks o
 * ``run(n, graph=True)`` replays HIP graphs that hold ``block`` batched steps (a multiple of 3, so
   the rotating state roles come back to the start) and a device step counter increment; the
   remainder runs as eager launches;
-* ``autotune()`` times candidate lane widths (two samples per lane where the count is even, or
-  one everywhere: fewer registers, more waves in flight) and persistent grid sizes (from a few
-  wavefronts per SIMD up to one wavefront per task) and keeps the fastest;
+* ``autotune()`` times candidate lane widths (two samples per lane where the count is even; one
+  everywhere; two only where the rows are wave-uniform) and grid sizes (persistent grids of a few
+  wavefronts per SIMD up to one wavefront per task) and keeps the fastest.  The launch uses a
+  kernel instance holding only the paths the plan needs (operators, damping, lane widths), so
+  the registers, and with them the waves in flight, follow the ensemble;
 * ``run_per_mesh(n, method)`` is the comparison path: each mesh's own single-mesh step kernel
   launched per mesh per step (with its own operator, or the one given).
 """
@@ -43,7 +45,10 @@ class BatchedEnsemble:
         self.device = torch.device(device) if device is not None else self.systems[0].device
         self.lib = _lib.lib()
         self.n_waves = 0
-        self.vw_hint = 0          # 0: two samples per lane for even counts, 1: one everywhere
+        # lanes: 0 two samples per lane for even counts, 1 one everywhere, 2 two only where the
+        # rows are wave-uniform with two (packed rows take one)
+        self.vw_hint = 0
+        self.cap = False          # cap the kernel at 72 registers (more waves, possible spills)
         self.kinds = 0
         self.phase = 0
         self.step_index = 0
@@ -87,6 +92,9 @@ class BatchedEnsemble:
         self.kinds = int(kinds.value)
         self._drop_graphs()
 
+    def _kinds(self):
+        return self.kinds | (32 if self.cap else 0)
+
     def _drop_graphs(self):
         for g in self._graphs.values():
             self.lib.cvfes_graph_destroy(g)
@@ -103,17 +111,17 @@ class BatchedEnsemble:
         for _ in range(n):
             check(self.lib.cvfes_batch_step(self.stream, self.dev_meshes.data_ptr(),
                                             len(self.systems), self.n_tasks, self.phase, None,
-                                            self.step_index, self.n_waves, self.kinds))
+                                            self.step_index, self.n_waves, self._kinds()))
             self.phase = (self.phase + 1) % 3
             self.step_index += 1
 
     def _graph(self, block):
-        key = (block, self.phase, self.n_waves, self.kinds)
+        key = (block, self.phase, self.n_waves, self._kinds())
         if key not in self._graphs:
             g = ctypes.c_void_p()
             check(self.lib.cvfes_batch_graph_create(self.dev_meshes.data_ptr(), len(self.systems),
                                                     self.n_tasks, self.counter.data_ptr(), block,
-                                                    self.phase, self.n_waves, self.kinds,
+                                                    self.phase, self.n_waves, self._kinds(),
                                                     ctypes.byref(g)))
             self._graphs[key] = g.value
         return self._graphs[key]
@@ -149,8 +157,8 @@ class BatchedEnsemble:
         self.phase = 0
 
     # ------------------------------------------------------------------------------------------
-    def autotune(self, candidates=None, steps=6, lane_widths=(0, 1)):
-        """Pick the lane width and persistent grid size (wavefronts) with the shortest step."""
+    def autotune(self, candidates=None, steps=6, lane_widths=(0, 1, 2), caps=(False, True)):
+        """Pick the lane width, register cap and grid size (wavefronts) with the shortest step."""
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         best = None
         saved = (self.phase, self.step_index)
@@ -158,24 +166,26 @@ class BatchedEnsemble:
             self.vw_hint = vw
             self._sync_roles()
             self._replan()
-            grid = candidates if candidates is not None else \
-                [cus * w for w in (8, 16, 32, 64, 128)] + [self.n_tasks]
-            for w in grid:
-                self.n_waves = w
-                self._eager(1)
-                torch.cuda.synchronize(self.device)
-                t0 = time.perf_counter()
-                self._eager(steps)
-                torch.cuda.synchronize(self.device)
-                dt = (time.perf_counter() - t0) / steps
-                if best is None or dt < best[2]:
-                    best = (vw, w, dt)
-        self.vw_hint, self.n_waves = best[0], best[1]
+            grid = candidates if candidates is not None else [cus * 16, cus * 64, self.n_tasks]
+            for cap in caps:
+                self.cap = cap
+                for w in grid:
+                    self.n_waves = w
+                    self._eager(1)
+                    torch.cuda.synchronize(self.device)
+                    t0 = time.perf_counter()
+                    self._eager(steps)
+                    torch.cuda.synchronize(self.device)
+                    dt = (time.perf_counter() - t0) / steps
+                    if best is None or dt < best[3]:
+                        best = (vw, cap, w, dt)
+        self.vw_hint, self.cap, self.n_waves = best[0], best[1], best[2]
         # the tuning steps advanced the state: restart the bookkeeping from the current roles
         self._sync_roles()
         self._replan()
-        self.tuned = {"lane_width": "one sample per lane" if best[0] == 1 else "two where even",
-                      "n_waves": best[1], "ms_per_step": best[2] * 1e3, "from_step": saved[1]}
+        names = {0: "two where even", 1: "one everywhere", 2: "two where rows are uniform"}
+        self.tuned = {"lane_width": names[best[0]], "register_cap": best[1], "n_waves": best[2],
+                      "ms_per_step": best[3] * 1e3, "from_step": saved[1]}
         return self.tuned
 
     def _replan(self):

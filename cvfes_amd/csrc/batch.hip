@@ -23,33 +23,44 @@
 namespace cvfes {
 namespace {
 
+// The descriptors' pointers arrive through memory, so the compiler cannot tell that they point to
+// global memory and would use flat loads with 64-bit vector addresses; the row step gets a view
+// with global-address-space pointers instead (global loads with scalar bases, fewer registers).
+template <class T>
+__device__ __forceinline__ const CVFES_G T *as_global(const T *p) {
+    return (const CVFES_G T *)p;
+}
+
 template <bool MF, bool BETA, bool DAMP, int VW>
 __device__ __forceinline__ void batch_task(const cvfes_batch_mesh_t &d, int phase, int64_t n,
                                            int32_t row, int64_t s, bool active) {
-    StepArgs a;
-    a.k = RowArgs{d.sys.n_nodes, d.sys.n_smp, d.sys.indptr, d.sys.indices, d.sys.values,
-                  d.sys.cptr,    d.sys.celem, d.sys.khat,   d.sys.coef};
-    a.u = d.state[phase];
-    a.up = d.state[(phase + 2) % 3];
-    a.out = d.state[(phase + 1) % 3];
-    a.lm = d.sys.lm;
+    GStepArgs a;
+    a.k = GRowArgs{d.sys.n_nodes, d.sys.n_smp, as_global(d.sys.indptr), as_global(d.sys.indices),
+                   as_global(d.sys.values), as_global(d.sys.cptr), as_global(d.sys.celem),
+                   as_global(d.sys.khat), as_global(d.sys.coef)};
+    a.u = as_global(d.state[phase]);
+    a.up = as_global(d.state[(phase + 2) % 3]);
+    a.out = (CVFES_G double *)d.state[(phase + 1) % 3];
+    a.lm = as_global(d.sys.lm);
     a.lm_sstride = d.sys.lm_sstride;
-    a.f1 = d.sys.f1;
-    a.fixed = d.sys.fixed;
-    a.load = d.load;
+    a.f1 = as_global(d.sys.f1);
+    a.fixed = as_global(d.sys.fixed);
+    a.load = GLoad{as_global(d.load.p), as_global(d.load.ga), as_global(d.load.qa),
+                   as_global(d.load.gb), as_global(d.load.qb)};
     a.bdy_u = d.sys.bdy_u;
     a.dt2 = d.dt * d.dt;
     a.bdt = d.beta / d.dt;
     a.ddt = d.damp / d.dt;
-    a.order = nullptr;
     step_row<MF, BETA, DAMP, VW>(a, row, s, active, n);
 }
 
-template <int VW>
+// Path sets: a kernel instance only contains the operators and damping forms that occur in the
+// ensemble (cvfes_batch_plan reports them), so it needs no more registers than those paths do.
+template <int VW, bool STORED, bool BD>
 __device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int phase, int64_t n,
                                                int32_t row, int64_t s, bool active) {
-    const bool mf = d.method == CVFES_METHOD_MATFREE;
-    const bool beta = d.beta != 0.0, damp = d.damp != 0.0;
+    const bool mf = !STORED || d.method == CVFES_METHOD_MATFREE;
+    const bool beta = BD && d.beta != 0.0, damp = BD && d.damp != 0.0;
     if (mf) {
         if (beta) {
             if (damp) batch_task<true, true, true, VW>(d, phase, n, row, s, active);
@@ -58,7 +69,7 @@ __device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int 
             if (damp) batch_task<true, false, true, VW>(d, phase, n, row, s, active);
             else batch_task<true, false, false, VW>(d, phase, n, row, s, active);
         }
-    } else {
+    } else if constexpr (STORED) {
         if (beta) {
             if (damp) batch_task<false, true, true, VW>(d, phase, n, row, s, active);
             else batch_task<false, true, false, VW>(d, phase, n, row, s, active);
@@ -69,7 +80,7 @@ __device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int 
     }
 }
 
-template <bool HAS_VW2, bool HAS_VW1>
+template <bool ONE, bool U2, bool P2, bool V1, bool STORED, bool BD>
 __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh,
                                            int64_t n_tasks, int phase,
                                            const int64_t *__restrict__ counter, int64_t k_local) {
@@ -82,7 +93,8 @@ __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict_
         static_cast<int32_t>((blockIdx.x / kNumXcd) * kWavesPerBlock + threadIdx.x / 64));
     const int lane = threadIdx.x & 63;
     const int64_t items = n_tasks / kNumXcd;
-    const int64_t per = (items + waves_x - 1) / waves_x;
+    // ONE: the grid has a wavefront per item (no loop, fewer registers)
+    const int64_t per = ONE ? 1 : (items + waves_x - 1) / waves_x;
     const int64_t i0 = lw * per;
     const int64_t i1 = i0 + per < items ? i0 + per : items;
     if (i0 >= i1) return;
@@ -113,42 +125,56 @@ __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict_
             const bool active = in && s < S;
             const int32_t row = in ? static_cast<int32_t>(r) : static_cast<int32_t>(d.sys.n_nodes - 1);
             if (!active) s = S - vw;
-            if (HAS_VW2 && (!HAS_VW1 || vw == 2)) batch_dispatch<2>(d, phase, n, row, s, active);
-            else if (HAS_VW1) batch_dispatch<1>(d, phase, n, row, s, active);
+            if (P2 && (!V1 || vw == 2)) batch_dispatch<2, STORED, BD>(d, phase, n, row, s, active);
+            else if (V1) batch_dispatch<1, STORED, BD>(d, phase, n, row, s, active);
         } else {
             const int32_t row = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(g / lanes));
             int64_t s = static_cast<int64_t>(g - static_cast<uint32_t>(row) * lanes) * vw;
             const bool active = s < S;
             if (!active) s = S - vw;
-            if (HAS_VW2 && (!HAS_VW1 || vw == 2)) batch_dispatch<2>(d, phase, n, row, s, active);
-            else if (HAS_VW1) batch_dispatch<1>(d, phase, n, row, s, active);
+            if (U2 && (!V1 || vw == 2)) batch_dispatch<2, STORED, BD>(d, phase, n, row, s, active);
+            else if (V1) batch_dispatch<1, STORED, BD>(d, phase, n, row, s, active);
         }
     }
 }
 
-template <bool HAS_VW2, bool HAS_VW1>
-__global__ __launch_bounds__(kBlock) void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes,
-                                                        int n_mesh, int64_t n_tasks, int phase,
-                                                        const int64_t *__restrict__ counter,
-                                                        int64_t k_local) {
-    batch_body<HAS_VW2, HAS_VW1>(meshes, n_mesh, n_tasks, phase, counter, k_local);
-}
-
-// One sample per lane everywhere: capped at 72 registers (7 waves per SIMD, no spills), against
-// 80 registers (6 waves) uncapped.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7)))
-void k_batch_step_vw1(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh, int64_t n_tasks,
-                      int phase, const int64_t *__restrict__ counter, int64_t k_local) {
-    batch_body<false, true>(meshes, n_mesh, n_tasks, phase, counter, k_local);
+// CAP: at most 72 registers (7 waves per SIMD), with spills where the paths need more; the
+// autotuner decides whether the extra waves pay for the spills.
+template <bool ONE, bool U2, bool P2, bool V1, bool STORED, bool BD, bool CAP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP ? 7 : 1)))
+void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh, int64_t n_tasks,
+                  int phase, const int64_t *__restrict__ counter, int64_t k_local) {
+    batch_body<ONE, U2, P2, V1, STORED, BD>(meshes, n_mesh, n_tasks, phase, counter, k_local);
 }
 
 using BatchKernel = void (*)(const cvfes_batch_mesh_t *, int, int64_t, int, const int64_t *, int64_t);
 
-BatchKernel batch_kernel(int64_t kinds) {
-    const bool v2 = kinds & 1, v1 = kinds & 2;
-    if (v2 && v1) return k_batch_step<true, true>;
-    if (v2) return k_batch_step<true, false>;
-    return k_batch_step_vw1;
+// Smallest instance whose lane paths cover the plan's: (uniform two-sample, packed two-sample,
+// one-sample) in {(1,1,1), (1,1,0), (1,0,1), (1,0,0), (0,0,1)}.
+template <bool ONE, bool STORED, bool BD, bool CAP>
+BatchKernel batch_kernel_lanes(int64_t kinds) {
+    const bool u2 = kinds & 1, v1 = kinds & 2, p2 = kinds & 16;
+    if (p2) return v1 ? k_batch_step<ONE, true, true, true, STORED, BD, CAP>
+                      : k_batch_step<ONE, true, true, false, STORED, BD, CAP>;
+    if (u2) return v1 ? k_batch_step<ONE, true, false, true, STORED, BD, CAP>
+                      : k_batch_step<ONE, true, false, false, STORED, BD, CAP>;
+    return k_batch_step<ONE, false, false, true, STORED, BD, CAP>;
+}
+
+template <bool ONE, bool CAP>
+BatchKernel batch_kernel_sb(int64_t kinds) {
+    const bool st = kinds & 4, bd = kinds & 8;
+    if (st) return bd ? batch_kernel_lanes<ONE, true, true, CAP>(kinds) : batch_kernel_lanes<ONE, true, false, CAP>(kinds);
+    return bd ? batch_kernel_lanes<ONE, false, true, CAP>(kinds) : batch_kernel_lanes<ONE, false, false, CAP>(kinds);
+}
+
+// kinds (from cvfes_batch_plan): bit 0 some wave-uniform mesh with two samples per lane, bit 1
+// some mesh with one sample per lane, bit 2 some stored mesh, bit 3 some mesh with beta or damp,
+// bit 4 some packed mesh with two samples per lane; bit 5 (set by the caller) caps the registers.
+BatchKernel batch_kernel(int64_t kinds, bool one) {
+    const bool cap = kinds & 32;
+    if (one) return cap ? batch_kernel_sb<true, true>(kinds) : batch_kernel_sb<true, false>(kinds);
+    return cap ? batch_kernel_sb<false, true>(kinds) : batch_kernel_sb<false, false>(kinds);
 }
 
 __global__ void k_counter_add(int64_t *counter, int64_t by) {
@@ -197,11 +223,22 @@ extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_
                       "cvfes_batch_plan: mesh %d load tables", i);
         CVFES_REQUIRE(m.dt > 0.0, "cvfes_batch_plan: mesh %d dt", i);
         CVFES_REQUIRE(s.lm_sstride == 0 || s.lm_sstride == 1, "cvfes_batch_plan: mesh %d lm_sstride", i);
-        CVFES_REQUIRE(m.vw == 0 || m.vw == 1 || m.vw == 2, "cvfes_batch_plan: mesh %d vw hint", i);
-        m.vw = (m.vw != 1 && s.n_smp % 2 == 0) ? 2 : 1;
-        k |= m.vw == 2 ? 1 : 2;
-        const int64_t l = s.n_smp / m.vw, padded = (l + 63) / 64 * 64;
+        CVFES_REQUIRE(m.vw >= 0 && m.vw <= 3, "cvfes_batch_plan: mesh %d vw hint", i);
+        // hint: 0 two samples per lane for even counts, 1 one everywhere, 2 two samples only where
+        // the rows are wave-uniform with two (packed rows take one)
+        const int64_t hint = m.vw;
+        m.vw = (hint != 1 && s.n_smp % 2 == 0) ? 2 : 1;
+        int64_t l = s.n_smp / m.vw, padded = (l + 63) / 64 * 64;
         m.packed = 4 * l >= 3 * padded ? 0 : 1;
+        if (hint == 2 && m.vw == 2 && m.packed) {
+            m.vw = 1;
+            l = s.n_smp;
+            padded = (l + 63) / 64 * 64;
+            m.packed = 4 * l >= 3 * padded ? 0 : 1;
+        }
+        k |= m.vw == 1 ? 2 : (m.packed ? 16 : 1);
+        if (m.method == CVFES_METHOD_STORED) k |= 4;
+        if (m.beta != 0.0 || m.damp != 0.0) k |= 8;
         m.lanes = m.packed ? l : padded;
         CVFES_REQUIRE(s.n_nodes * m.lanes < (int64_t(1) << 31) - 64,
                       "cvfes_batch_plan: mesh %d has too many row lanes", i);
@@ -227,10 +264,10 @@ extern "C" int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh
                                 int64_t n_waves, int64_t kinds) {
     CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && n_tasks >= 0, "cvfes_batch_step: bad arguments");
     CVFES_REQUIRE(phase >= 0 && phase < 3, "cvfes_batch_step: phase must be 0, 1 or 2");
-    CVFES_REQUIRE(kinds >= 1 && kinds <= 3, "cvfes_batch_step: kinds must come from cvfes_batch_plan");
+    CVFES_REQUIRE(kinds >= 1 && kinds <= 63 && (kinds & 19), "cvfes_batch_step: kinds must come from cvfes_batch_plan");
     if (n_tasks == 0) return CVFES_OK;
     const int64_t w = n_waves > 0 ? n_waves : default_waves();
-    hipLaunchKernelGGL(batch_kernel(kinds), dim3(waves_to_blocks(w)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(batch_kernel(kinds, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock), 0,
                        static_cast<hipStream_t>(stream),
                        static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks, phase,
                        counter, k_local);
@@ -247,7 +284,7 @@ extern "C" int cvfes_counter_add(void *stream, int64_t *counter, int64_t by) {
 extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int64_t n_tasks,
                                         int64_t *counter, int64_t block_steps, int phase0,
                                         int64_t n_waves, int64_t kinds, void **graph) {
-    CVFES_REQUIRE(kinds >= 1 && kinds <= 3, "cvfes_batch_graph_create: kinds must come from cvfes_batch_plan");
+    CVFES_REQUIRE(kinds >= 1 && kinds <= 63 && (kinds & 19), "cvfes_batch_graph_create: kinds must come from cvfes_batch_plan");
     CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && counter && graph, "cvfes_batch_graph_create: bad arguments");
     CVFES_REQUIRE(block_steps >= 3 && block_steps % 3 == 0,
                   "cvfes_batch_graph_create: block_steps must be a positive multiple of 3");
@@ -262,7 +299,7 @@ extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int6
         return fail(CVFES_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(err));
     }
     for (int64_t k = 0; k < block_steps; ++k)
-        hipLaunchKernelGGL(batch_kernel(kinds), dim3(waves_to_blocks(w)), dim3(kBlock), 0, cs,
+        hipLaunchKernelGGL(batch_kernel(kinds, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock), 0, cs,
                            static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks,
                            static_cast<int>((phase0 + k) % 3), counter, k);
     hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, cs, counter, block_steps);

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(StepArgs a) {
     bool active;
     if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
     double acc[3][VW];
-    row_product<MATFREE, false, VW>(a.k, a.u, nullptr, 0.0, row, s, acc);
+    row_product<MATFREE, false, VW>(a.k, a.u, static_cast<const double *>(nullptr), 0.0, row, s, acc);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         Vec<VW> o;

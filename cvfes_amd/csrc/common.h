@@ -87,4 +87,35 @@ __device__ __forceinline__ void store_vec(double *__restrict__ p, const Vec<VW> 
     }
 }
 
+// Pointers typed into the global address space.  Kernels whose array pointers arrive through
+// memory (the batched descriptors) use them so that the compiler emits global loads with scalar
+// bases instead of flat loads with 64-bit vector addresses.
+#define CVFES_G __attribute__((address_space(1)))
+
+typedef double gdbl2_t __attribute__((ext_vector_type(2)));
+
+template <int VW>
+__device__ __forceinline__ Vec<VW> load_vec(const CVFES_G double *__restrict__ p) {
+    Vec<VW> r;
+    if constexpr (VW == 2) {
+        const gdbl2_t t = *reinterpret_cast<const CVFES_G gdbl2_t *>(p);
+        r.v[0] = t.x;
+        r.v[1] = t.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) r.v[k] = p[k];
+    }
+    return r;
+}
+
+template <int VW>
+__device__ __forceinline__ void store_vec(CVFES_G double *__restrict__ p, const Vec<VW> &r) {
+    if constexpr (VW == 2) {
+        *reinterpret_cast<CVFES_G gdbl2_t *>(p) = gdbl2_t{r.v[0], r.v[1]};
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) p[k] = r.v[k];
+    }
+}
+
 }  // namespace cvfes

@@ -67,11 +67,45 @@ struct StepArgs {
     const int32_t *order; // visiting order of the rows (wave-uniform kernels; null = natural)
 };
 
-template <bool MATFREE, int VW>
-__device__ __forceinline__ void block_values(const RowArgs &k, int32_t p, int64_t s,
+// The same arguments with global-address-space pointers (see CVFES_G); the generic row functions
+// below take either form.
+struct GRowArgs {
+    int64_t n_nodes;
+    int64_t n_smp;
+    const CVFES_G int32_t *indptr;
+    const CVFES_G int32_t *indices;
+    const CVFES_G double *values;
+    const CVFES_G int32_t *cptr;
+    const CVFES_G int32_t *celem;
+    const CVFES_G double *khat;
+    const CVFES_G double *coef;
+};
+
+struct GLoad {
+    const CVFES_G double *p, *ga, *qa, *gb, *qb;
+};
+
+struct GStepArgs {
+    GRowArgs k;
+    const CVFES_G double *u;
+    const CVFES_G double *up;
+    CVFES_G double *out;
+    const CVFES_G double *lm;
+    int64_t lm_sstride;
+    const CVFES_G double *f1;
+    const CVFES_G uint8_t *fixed;
+    GLoad load;
+    double bdy_u;
+    double dt2;
+    double bdt;
+    double ddt;
+};
+
+template <bool MATFREE, int VW, class RA>
+__device__ __forceinline__ void block_values(const RA &k, int32_t p, int64_t s,
                                              double (&kv)[9][VW]) {
     if constexpr (!MATFREE) {
-        const double *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
+        const auto *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
 #pragma unroll
         for (int ij = 0; ij < 9; ++ij) {
             const Vec<VW> t = load_vec<VW>(base + ij * k.n_smp);
@@ -87,7 +121,7 @@ __device__ __forceinline__ void block_values(const RowArgs &k, int32_t p, int64_
         for (int32_t c = c0; c < c1; ++c) {
             const int64_t e = k.celem[c];
             const Vec<VW> cf = load_vec<VW>(k.coef + e * k.n_smp + s);
-            const double *kh = k.khat + 9 * static_cast<int64_t>(c);
+            const auto *kh = k.khat + 9 * static_cast<int64_t>(c);
 #pragma unroll
             for (int ij = 0; ij < 9; ++ij) {
                 const double h = kh[ij];
@@ -99,9 +133,8 @@ __device__ __forceinline__ void block_values(const RowArgs &k, int32_t p, int64_
 }
 
 // acc[i][v] = sum over the row's blocks of K_p (i, :) . x_col, with x = u + bdt (u - up) when BETA.
-template <bool MATFREE, bool BETA, int VW>
-__device__ __forceinline__ void row_product(const RowArgs &k, const double *__restrict__ u,
-                                            const double *__restrict__ up, double bdt, int32_t row,
+template <bool MATFREE, bool BETA, int VW, class RA, class DP>
+__device__ __forceinline__ void row_product(const RA &k, DP u, DP up, double bdt, int32_t row,
                                             int64_t s, double (&acc)[3][VW]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -137,8 +170,8 @@ __device__ __forceinline__ void row_product(const RowArgs &k, const double *__re
 
 // Block-row product, central-difference update, load and Dirichlet value for one block row and
 // VW consecutive samples; stores u_{n+1} when `active`.
-template <bool MATFREE, bool BETA, bool DAMP, int VW>
-__device__ __forceinline__ void step_row(const StepArgs &a, int32_t row, int64_t s, bool active,
+template <bool MATFREE, bool BETA, bool DAMP, int VW, class SA>
+__device__ __forceinline__ void step_row(const SA &a, int32_t row, int64_t s, bool active,
                                          int64_t step) {
     double acc[3][VW];
     row_product<MATFREE, BETA, VW>(a.k, a.u, a.up, a.bdt, row, s, acc);
@@ -147,7 +180,7 @@ __device__ __forceinline__ void step_row(const StepArgs &a, int32_t row, int64_t
     const double qb = a.load.gb ? a.load.qb[step] : 0.0;
     const bool fixed = a.fixed != nullptr && a.fixed[row] != 0;
     const int64_t nsm = a.k.n_smp;
-    const double *lmp = a.lm + (a.lm_sstride ? static_cast<int64_t>(row) * nsm + s : row);
+    const auto *lmp = a.lm + (a.lm_sstride ? static_cast<int64_t>(row) * nsm + s : row);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int64_t off = (static_cast<int64_t>(row) * 3 + i) * nsm + s;

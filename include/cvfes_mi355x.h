@@ -296,8 +296,9 @@ typedef struct cvfes_batch_mesh {
     double dt, beta, damp;
     int64_t task0;         /* set by cvfes_batch_plan: first wave task of the mesh             */
     int64_t lanes;         /* set by cvfes_batch_plan: lanes per block row                     */
-    int64_t vw;            /* samples per lane: 0 on input = choose (2 for even counts, else 1),
-                              1 = one sample per lane; set by cvfes_batch_plan                 */
+    int64_t vw;            /* samples per lane.  On input: 0 = two for even counts, 1 = one
+                              everywhere, 2 = two only where the rows are wave-uniform with two;
+                              set by cvfes_batch_plan to the width used (1 or 2)             */
     int64_t packed;        /* set by cvfes_batch_plan: 1 = rows packed back to back in lanes,
                               0 = each row padded to whole wavefronts                         */
     int64_t n_mtasks;      /* set by cvfes_batch_plan: wave tasks of the mesh                   */
@@ -313,10 +314,12 @@ typedef struct cvfes_batch_mesh {
  * cut into 8 contiguous slices, one per XCD, and each XCD walks its slice of every mesh in turn, so
  * the XCDs get equal shares of every mesh (balanced) and contiguous rows (neighbour states shared in
  * their L2).  n_tasks receives 8 x the slice items of all meshes (the size of the task space; one
- * wavefront per task is n_waves = n_tasks).  kinds
- * receives which lane widths occur (bit 0: vw 2, bit 1: vw 1); pass it to cvfes_batch_step and
- * cvfes_batch_graph_create, which then launch a kernel instance with only those paths (fewer
- * registers, more waves in flight).  Blocks until the copy is done. */
+ * wavefront per task is n_waves = n_tasks).  kinds receives the paths the ensemble needs (bit 0:
+ * wave-uniform rows with two samples per lane, bit 1: one sample per lane, bit 2: a stored mesh,
+ * bit 3: beta or damp, bit 4: packed rows with two samples per lane); pass it to cvfes_batch_step
+ * and cvfes_batch_graph_create, which then launch a kernel instance with only those paths (fewer
+ * registers, more waves in flight; n_waves >= n_tasks selects the instance without the persistent
+ * loop).  Blocks until the copy is done. */
 int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void *dev_meshes,
                      int64_t *n_tasks, int64_t *kinds);
 

@@ -46,19 +46,19 @@ def states(systems):
     return [s.u.cpu().numpy().copy() for s in systems]
 
 
-@pytest.mark.parametrize("vw_hint", [0, 1])
-def test_batched_equals_per_mesh(require_gpu, vw_hint):
-    """Both lane widths (two samples per lane where even / one everywhere) give the per-mesh
-    result bit for bit."""
+@pytest.mark.parametrize("vw_hint,cap", [(0, False), (1, False), (2, False), (2, True), (0, True)])
+def test_batched_equals_per_mesh(require_gpu, vw_hint, cap):
+    """Every lane-width policy (two samples per lane where even / one everywhere / two only on
+    wave-uniform rows) gives the per-mesh result bit for bit."""
     from cvfes_amd.batch import BatchedEnsemble
     nsteps = 40
     systems, loads = make_ensemble(nsteps)
     cols = list(zip(*SPECS))
     ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
                           betas=list(cols[6]), damps=list(cols[7]))
-    ens.vw_hint = vw_hint
+    ens.vw_hint, ens.cap = vw_hint, cap
     ens.initial_state()
-    assert ens.kinds == (3 if vw_hint == 0 else 2)
+    assert ens.kinds == {0: 31, 1: 14, 2: 15}[vw_hint]
     ens.run(nsteps, graph=False)
     batched = states(systems)
     ens.initial_state()
@@ -146,7 +146,7 @@ def test_autotune_keeps_the_trajectory(require_gpu):
     ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
                           betas=list(cols[6]), damps=list(cols[7]))
     ens.initial_state()
-    tuned = ens.autotune(candidates=[256, 2048], steps=2)
+    tuned = ens.autotune(candidates=[256, 2048], steps=1)
     assert tuned["n_waves"] in (256, 2048)
     done = ens.step_index
     ens.run(nsteps - done, graph=True, block=33)
