@@ -106,6 +106,7 @@ _SIGNATURES = {
     "cvfes_step_updated": (_int, [_vp, ctypes.POINTER(CvfesSystem), ctypes.POINTER(CvfesGeometry),
                                   ctypes.POINTER(CvfesUpdate), ctypes.POINTER(_vp),
                                   ctypes.POINTER(CvfesLoad), _i64, _f64, _f64, _f64]),
+    "cvfes_batch_plan_bytes": (_i64, [_int]),
     "cvfes_batch_plan": (_int, [_vp, ctypes.POINTER(CvfesBatchMesh), _int, _vp,
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "cvfes_batch_step": (_int, [_vp, _vp, _int, _i64, _int, _vp, _i64, _i64, _i64]),

@@ -83,7 +83,8 @@ class BatchedEnsemble:
             m.dt, m.beta, m.damp = self.dts[i], self.betas[i], self.damps[i]
             m.vw = self.vw_hint
         self._bufs = [(s.state[0], s.state[2], s.state[1]) for s in self.systems]   # state[0..2]
-        self.dev_meshes = torch.empty(ctypes.sizeof(arr), dtype=torch.uint8, device=self.device)
+        nbytes = int(self.lib.cvfes_batch_plan_bytes(n))
+        self.dev_meshes = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         n_tasks, kinds = ctypes.c_int64(0), ctypes.c_int64(0)
         check(self.lib.cvfes_batch_plan(self.stream, arr, n, self.dev_meshes.data_ptr(),
                                         ctypes.byref(n_tasks), ctypes.byref(kinds)))

@@ -19,6 +19,7 @@
 #include "rowops.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace cvfes {
 namespace {
@@ -100,16 +101,17 @@ __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict_
     if (i0 >= i1) return;
     const int64_t n = (counter ? counter[0] : 0) + k_local;
     // Mesh of the first item: the number of meshes with xtask0 <= i0, minus one.  Each lane reads
-    // one descriptor's xtask0 (all loads of the wave issue together, 64 meshes per round) and a
-    // ballot counts them.
+    // one mesh's xtask0 and a ballot counts them.
+    // xtask0 of every mesh, packed after the descriptors (one coalesced load per 64 meshes)
+    const int64_t *__restrict__ xt = reinterpret_cast<const int64_t *>(meshes + n_mesh);
     int mi = -1;
     for (int base = 0; base < n_mesh; base += 64) {
         const int m = base + lane;
-        const bool le = m < n_mesh && meshes[m].xtask0 <= i0;
+        const bool le = m < n_mesh && xt[m] <= i0;
         mi += __popcll(__ballot(le));
     }
     for (int64_t i = i0; i < i1; ++i) {
-        while (mi + 1 < n_mesh && i >= meshes[mi + 1].xtask0) ++mi;
+        while (mi + 1 < n_mesh && i >= xt[mi + 1]) ++mi;
         const cvfes_batch_mesh_t &d = meshes[mi];
         const int64_t local = xcd * d.slice + (i - d.xtask0);    // task within the mesh
         if (local >= d.n_mtasks) continue;                       // the last slice is short
@@ -202,6 +204,10 @@ int default_waves() {
 
 using namespace cvfes;
 
+extern "C" int64_t cvfes_batch_plan_bytes(int n_mesh) {
+    return n_mesh < 1 ? -1 : static_cast<int64_t>(n_mesh) * (sizeof(cvfes_batch_mesh_t) + sizeof(int64_t));
+}
+
 extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh,
                                 void *dev_meshes, int64_t *n_tasks, int64_t *kinds) {
     CVFES_REQUIRE(meshes && n_mesh >= 1 && dev_meshes && n_tasks && kinds,
@@ -252,8 +258,13 @@ extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_
     *n_tasks = x * kNumXcd;
     *kinds = k;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<int64_t> xt(n_mesh);
+    for (int i = 0; i < n_mesh; ++i) xt[i] = meshes[i].xtask0;
     hipError_t err = hipMemcpyAsync(dev_meshes, meshes, sizeof(cvfes_batch_mesh_t) * n_mesh,
                                     hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(static_cast<char *>(dev_meshes) + sizeof(cvfes_batch_mesh_t) * n_mesh,
+                             xt.data(), sizeof(int64_t) * n_mesh, hipMemcpyHostToDevice, st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
     if (err != hipSuccess) return fail(CVFES_ERR_HIP, "cvfes_batch_plan: %s", hipGetErrorString(err));
     return CVFES_OK;

@@ -306,8 +306,13 @@ typedef struct cvfes_batch_mesh {
     int64_t xtask0;        /* set by cvfes_batch_plan: first item of the mesh in every XCD's list */
 } cvfes_batch_mesh_t;
 
-/* Number the wave tasks of `meshes` (host array, updated in place) and copy the array to
- * `dev_meshes` (device, n_mesh * sizeof(cvfes_batch_mesh_t) bytes).  A wave task is 64 lanes of
+/* Device bytes cvfes_batch_plan writes for n_mesh meshes: the descriptors followed by a compact
+ * table of their xtask0 (-1 if n_mesh < 1). */
+int64_t cvfes_batch_plan_bytes(int n_mesh);
+
+/* Number the wave tasks of `meshes` (host array, updated in place) and copy the array, followed by
+ * a compact table of the meshes' xtask0, to `dev_meshes` (device, cvfes_batch_plan_bytes(n_mesh)
+ * bytes).  A wave task is 64 lanes of
  * one mesh, each lane vw consecutive samples of one block row.  A mesh whose sample count fills at
  * least 3/4 of whole wavefronts gets wave-uniform rows (lanes = S/vw rounded up to 64: row data
  * through scalar loads); otherwise its rows are packed back to back (lanes = S/vw).  Every mesh is
