@@ -32,22 +32,28 @@ __device__ __forceinline__ const CVFES_G T *as_global(const T *p) {
     return (const CVFES_G T *)p;
 }
 
+// Arrays the step only reads: constant address space, so wave-uniform reads are scalar loads.
+template <class T>
+__device__ __forceinline__ const CVFES_K T *as_const(const T *p) {
+    return (const CVFES_K T *)p;
+}
+
 template <bool MF, bool BETA, bool DAMP, int VW>
 __device__ __forceinline__ void batch_task(const cvfes_batch_mesh_t &d, int phase, int64_t n,
                                            int32_t row, int64_t s, bool active) {
     GStepArgs a;
-    a.k = GRowArgs{d.sys.n_nodes, d.sys.n_smp, as_global(d.sys.indptr), as_global(d.sys.indices),
-                   as_global(d.sys.values), as_global(d.sys.cptr), as_global(d.sys.celem),
-                   as_global(d.sys.khat), as_global(d.sys.coef)};
+    a.k = GRowArgs{d.sys.n_nodes, d.sys.n_smp, as_const(d.sys.indptr), as_const(d.sys.indices),
+                   as_global(d.sys.values), as_const(d.sys.cptr), as_const(d.sys.celem),
+                   as_const(d.sys.khat), as_global(d.sys.coef)};
     a.u = as_global(d.state[phase]);
     a.up = as_global(d.state[(phase + 2) % 3]);
     a.out = (CVFES_G double *)d.state[(phase + 1) % 3];
-    a.lm = as_global(d.sys.lm);
+    a.lm = as_const(d.sys.lm);
     a.lm_sstride = d.sys.lm_sstride;
-    a.f1 = as_global(d.sys.f1);
-    a.fixed = as_global(d.sys.fixed);
-    a.load = GLoad{as_global(d.load.p), as_global(d.load.ga), as_global(d.load.qa),
-                   as_global(d.load.gb), as_global(d.load.qb)};
+    a.f1 = as_const(d.sys.f1);
+    a.fixed = as_const(d.sys.fixed);
+    a.load = GLoad{as_const(d.load.p), as_const(d.load.ga), as_const(d.load.qa),
+                   as_const(d.load.gb), as_const(d.load.qb)};
     a.bdy_u = d.sys.bdy_u;
     a.dt2 = d.dt * d.dt;
     a.bdt = d.beta / d.dt;

@@ -92,6 +92,12 @@ __device__ __forceinline__ void store_vec(double *__restrict__ p, const Vec<VW> 
 // bases instead of flat loads with 64-bit vector addresses.
 #define CVFES_G __attribute__((address_space(1)))
 
+// Pointers typed into the constant address space, for arrays that no kernel writes while the
+// launch runs (structure, unit blocks, masses, loads).  A wave-uniform address then becomes a
+// scalar load without the compiler having to prove that no store of the kernel can reach it,
+// a proof it gives up on in large kernels; a divergent address is still a vector global load.
+#define CVFES_K __attribute__((address_space(4)))
+
 typedef double gdbl2_t __attribute__((ext_vector_type(2)));
 
 template <int VW>

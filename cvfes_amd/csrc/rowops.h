@@ -69,20 +69,22 @@ struct StepArgs {
 
 // The same arguments with global-address-space pointers (see CVFES_G); the generic row functions
 // below take either form.
+// Read-only per-row data (structure, unit blocks, masses, loads) is in the constant address space
+// (CVFES_K); the per-sample streams (values, coefficients, states) are global.
 struct GRowArgs {
     int64_t n_nodes;
     int64_t n_smp;
-    const CVFES_G int32_t *indptr;
-    const CVFES_G int32_t *indices;
+    const CVFES_K int32_t *indptr;
+    const CVFES_K int32_t *indices;
     const CVFES_G double *values;
-    const CVFES_G int32_t *cptr;
-    const CVFES_G int32_t *celem;
-    const CVFES_G double *khat;
+    const CVFES_K int32_t *cptr;
+    const CVFES_K int32_t *celem;
+    const CVFES_K double *khat;
     const CVFES_G double *coef;
 };
 
 struct GLoad {
-    const CVFES_G double *p, *ga, *qa, *gb, *qb;
+    const CVFES_K double *p, *ga, *qa, *gb, *qb;
 };
 
 struct GStepArgs {
@@ -90,10 +92,10 @@ struct GStepArgs {
     const CVFES_G double *u;
     const CVFES_G double *up;
     CVFES_G double *out;
-    const CVFES_G double *lm;
+    const CVFES_K double *lm;
     int64_t lm_sstride;
-    const CVFES_G double *f1;
-    const CVFES_G uint8_t *fixed;
+    const CVFES_K double *f1;
+    const CVFES_K uint8_t *fixed;
     GLoad load;
     double bdy_u;
     double dt2;
