@@ -48,7 +48,7 @@ class BatchedEnsemble:
         # lanes: 0 two samples per lane for even counts, 1 one everywhere, 2 two only where the
         # rows are wave-uniform with two (packed rows take one)
         self.vw_hint = 0
-        self.cap = False          # cap the kernel at 72 registers (more waves, possible spills)
+        self.cap = False          # cap the kernel at 64 registers (8 waves per SIMD, possible spills)
         self.kinds = 0
         self.phase = 0
         self.step_index = 0

@@ -36,30 +36,34 @@ struct AdjArgs {
     int project;
 };
 
+// acc[i][v] += K_p (i, :) . x for block p (K symmetric: the same block serves the transposed
+// product).  Matrix-free: acc += coef_e (khat_c x) over the block's element contributions, without
+// forming the block.
 template <bool MATFREE, int VW>
-__device__ __forceinline__ void adj_block_values(const AdjArgs &a, int32_t p, int64_t s,
-                                                 double (&kv)[9][VW]) {
+__device__ __forceinline__ void adj_block_apply(const AdjArgs &a, int32_t p, int64_t s,
+                                                const double (&x)[3][VW], double (&acc)[3][VW]) {
     if constexpr (!MATFREE) {
         const double *base = a.values + (static_cast<int64_t>(p) * 9) * a.n_smp + s;
 #pragma unroll
-        for (int ij = 0; ij < 9; ++ij) {
-            const Vec<VW> t = load_vec<VW>(base + ij * a.n_smp);
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
-        }
+            for (int j = 0; j < 3; ++j) {
+                const Vec<VW> kv = load_vec<VW>(base + (3 * i + j) * a.n_smp);
+#pragma unroll
+                for (int v = 0; v < VW; ++v) acc[i][v] = fma(kv.v[v], x[j][v], acc[i][v]);
+            }
     } else {
-#pragma unroll
-        for (int ij = 0; ij < 9; ++ij)
-#pragma unroll
-            for (int v = 0; v < VW; ++v) kv[ij][v] = 0.0;
         for (int32_t c = a.cptr[p]; c < a.cptr[p + 1]; ++c) {
             const Vec<VW> cf = load_vec<VW>(a.coef + static_cast<int64_t>(a.celem[c]) * a.n_smp + s);
             const double *kh = a.khat + 9 * static_cast<int64_t>(c);
 #pragma unroll
-            for (int ij = 0; ij < 9; ++ij) {
-                const double h = kh[ij];
+            for (int i = 0; i < 3; ++i) {
+                const double h0 = kh[3 * i], h1 = kh[3 * i + 1], h2 = kh[3 * i + 2];
 #pragma unroll
-                for (int v = 0; v < VW; ++v) kv[ij][v] = fma(cf.v[v], h, kv[ij][v]);
+                for (int v = 0; v < VW; ++v) {
+                    const double t = fma(h2, x[2][v], fma(h1, x[1][v], h0 * x[0][v]));
+                    acc[i][v] = fma(cf.v[v], t, acc[i][v]);
+                }
             }
         }
     }
@@ -118,14 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
                 x[j][v] = mc.v[v] != 0.0 ? comb * (a.dt2 / mc.v[v]) : 0.0;
             }
         }
-        double kv[9][VW];
-        adj_block_values<MATFREE, VW>(a, p, s, kv);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int v = 0; v < VW; ++v) acc[i][v] = fma(kv[3 * i + j][v], x[j][v], acc[i][v]);
+        adj_block_apply<MATFREE, VW>(a, p, s, x, acc);
     }
     const bool fixed = a.project && a.fixed != nullptr && a.fixed[row] != 0;
     const Vec<VW> m = mass_at<VW>(a, row, s);

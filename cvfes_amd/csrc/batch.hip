@@ -146,10 +146,10 @@ __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict_
     }
 }
 
-// CAP: at most 72 registers (7 waves per SIMD), with spills where the paths need more; the
+// CAP: at most 64 registers (8 waves per SIMD), with spills where the paths need more; the
 // autotuner decides whether the extra waves pay for the spills.
 template <bool ONE, bool U2, bool P2, bool V1, bool STORED, bool BD, bool CAP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP ? 7 : 1)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP ? 8 : 1)))
 void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh, int64_t n_tasks,
                   int phase, const int64_t *__restrict__ counter, int64_t k_local) {
     batch_body<ONE, U2, P2, V1, STORED, BD>(meshes, n_mesh, n_tasks, phase, counter, k_local);

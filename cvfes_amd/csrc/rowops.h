@@ -1,5 +1,5 @@
-// Row operations shared by the single-mesh and the batched step kernels: block values (stored or
-// rebuilt from element coefficients), the gathered block-row product and the fused
+// Row operations shared by the single-mesh and the batched step kernels: the gathered block-row
+// product (stored blocks, or element unit blocks scaled by the element coefficients) and the fused
 // central-difference update of one block row.
 #pragma once
 
@@ -103,32 +103,48 @@ struct GStepArgs {
     double ddt;
 };
 
+// Stored block p of the row: kv[ij][v] = K_p[ij] of sample s + v.
+template <int VW, class RA>
+__device__ __forceinline__ void stored_block(const RA &k, int32_t p, int64_t s, double (&kv)[9][VW]) {
+    const auto *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
+#pragma unroll
+    for (int ij = 0; ij < 9; ++ij) {
+        const Vec<VW> t = load_vec<VW>(base + ij * k.n_smp);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
+    }
+}
+
+// acc[i][v] += K_p (i, :) . x for block p of the row.  Stored: the 3x3 block is read.  Matrix-free:
+// K_p = sum over the block's element contributions c of coef_e(c) * khat_c, applied as
+// acc += coef_e (khat_c x), so the block itself is never formed and only the three sums per
+// sample are live (khat_c is wave-uniform for wave-uniform rows: scalar loads).
 template <bool MATFREE, int VW, class RA>
-__device__ __forceinline__ void block_values(const RA &k, int32_t p, int64_t s,
-                                             double (&kv)[9][VW]) {
+__device__ __forceinline__ void block_apply(const RA &k, int32_t p, int64_t s,
+                                            const double (&x)[3][VW], double (&acc)[3][VW]) {
     if constexpr (!MATFREE) {
-        const auto *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
+        double kv[9][VW];
+        stored_block<VW>(k, p, s, kv);
 #pragma unroll
-        for (int ij = 0; ij < 9; ++ij) {
-            const Vec<VW> t = load_vec<VW>(base + ij * k.n_smp);
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
-        }
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int v = 0; v < VW; ++v) acc[i][v] = fma(kv[3 * i + j][v], x[j][v], acc[i][v]);
     } else {
-#pragma unroll
-        for (int ij = 0; ij < 9; ++ij)
-#pragma unroll
-            for (int v = 0; v < VW; ++v) kv[ij][v] = 0.0;
         const int32_t c0 = k.cptr[p], c1 = k.cptr[p + 1];
         for (int32_t c = c0; c < c1; ++c) {
             const int64_t e = k.celem[c];
             const Vec<VW> cf = load_vec<VW>(k.coef + e * k.n_smp + s);
             const auto *kh = k.khat + 9 * static_cast<int64_t>(c);
 #pragma unroll
-            for (int ij = 0; ij < 9; ++ij) {
-                const double h = kh[ij];
+            for (int i = 0; i < 3; ++i) {
+                const double h0 = kh[3 * i], h1 = kh[3 * i + 1], h2 = kh[3 * i + 2];
 #pragma unroll
-                for (int v = 0; v < VW; ++v) kv[ij][v] = fma(cf.v[v], h, kv[ij][v]);
+                for (int v = 0; v < VW; ++v) {
+                    const double t = fma(h2, x[2][v], fma(h1, x[1][v], h0 * x[0][v]));
+                    acc[i][v] = fma(cf.v[v], t, acc[i][v]);
+                }
             }
         }
     }
@@ -159,14 +175,7 @@ __device__ __forceinline__ void row_product(const RA &k, DP u, DP up, double bdt
                 for (int v = 0; v < VW; ++v) x[j][v] = xu.v[v];
             }
         }
-        double kv[9][VW];
-        block_values<MATFREE, VW>(k, p, s, kv);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int v = 0; v < VW; ++v) acc[i][v] = fma(kv[3 * i + j][v], x[j][v], acc[i][v]);
+        block_apply<MATFREE, VW>(k, p, s, x, acc);
     }
 }
 
