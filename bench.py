@@ -23,7 +23,8 @@ Other workloads (``--workload``):
            matrix-free steps timed, then 2,000 forward steps and the adjoint gradient with two-level
            checkpoints (every 200 steps, then every 20 inside a segment) and the observations in
            host memory, sized to fit 288 GB;
-  copy     a plain device copy of ``--copy-bytes`` (the calibration kernel for the counters).
+  copy     a plain device copy of ``--copy-bytes`` (the calibration kernel for the counters), timed:
+           the streaming rate the box reaches.
 """
 from __future__ import annotations
 
@@ -209,22 +210,24 @@ def main():
     }
     if not args.no_matfree:
         # the other operator and the symmetric storage of the stored one, measured the same way
-        alts = [("matfree", "auto")] if args.method == "stored" else [("stored", "auto")]
+        alts = [("matfree", "auto", "natural")] if args.method == "stored" else [("stored", "auto", "natural")]
         if args.method == "stored" and args.variant == "auto":
-            alts.append(("stored", "sym_ntl"))
+            alts += [("stored", "sym_ntl", "natural"), ("stored", "sym_ntl", "strips")]
         out["alternatives"] = []
-        for other, variant in alts:
+        for other, variant, order in alts:
             sysm.set_variant(variant)
+            sysm.set_row_order(order)
             msteps = max(20, args.steps // 2)
             mwall, mk = time_steps(sysm, load, dt, beta, other, args.warmup, msteps)
             mbytes = algorithmic_bytes(sysm, other)
             key = other if variant == "auto" else f"{other}-{variant}"
+            sysm.set_row_order("natural")
             out["alternatives"].append({
-                "operator": other, "variant": variant, "value": S * msteps / mwall,
+                "operator": other, "variant": variant, "row_order": order, "value": S * msteps / mwall,
                 "unit": "sample-steps/s", "kernel_ms": mk, "achieved_GBs": mbytes / (mk * 1e-3) / 1e9,
                 "frac": mbytes / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": mbytes,
                 "traffic": measured_traffic(case.name, key)})
-            print(f"[bench] {key}: {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
+            print(f"[bench] {key} ({order} order): {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
                   file=sys.stderr, flush=True)
         sysm.set_variant(args.variant)
     if not args.no_cpu:
@@ -257,7 +260,21 @@ def run_copy(args):
     for _ in range(5):
         torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    print(json.dumps({"workload": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 5}),
+    # the same launches timed with events (copy: read + write), and a read-only stream (sum)
+    st = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(st)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    e1.record(st)
+    for _ in range(5):
+        src.sum()
+    e2.record(st)
+    torch.cuda.synchronize()
+    t_copy = e0.elapsed_time(e1) / 5e3
+    t_sum = e1.elapsed_time(e2) / 5e3
+    print(json.dumps({"workload": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 5,
+                      "copy_GBps": 16 * n / t_copy / 1e9, "read_sum_GBps": 8 * n / t_sum / 1e9}),
           flush=True)
 
 

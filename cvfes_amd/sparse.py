@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["SparseInfo", "BlockMaps", "block_structure", "bisection_order"]
+__all__ = ["SparseInfo", "BlockMaps", "block_structure", "bisection_order", "strip_order"]
 
 
 def block_structure(elements: np.ndarray, n_nodes: int):
@@ -55,6 +55,39 @@ def bisection_order(nodes: np.ndarray, leaf: int = 64) -> np.ndarray:
         stack.append(o[h:])
         stack.append(o[:h])
     return np.concatenate(out).astype(np.int32) if out else np.zeros(0, np.int32)
+
+
+def strip_order(nodes: np.ndarray, elements: np.ndarray, width: int = 64) -> np.ndarray:
+    """Visiting order for tubular meshes: the wall is cut into angular strips about the tube axis,
+    each about ``width`` nodes of a ring wide, and every strip is swept along the axis (in bands of
+    half an edge length, by angle inside a band).  A node's neighbours in the next and previous
+    ring are then about ``width`` positions away instead of a whole ring, so a block read by one
+    row is read again by its mirror row soon after.
+
+    Axis: the principal direction of the node cloud.  Ring size: circumference (median distance to
+    the axis) over the median edge length."""
+    x = np.asarray(nodes, np.float64)
+    x = x - x.mean(0)
+    n = x.shape[0]
+    if n == 0:
+        return np.zeros(0, np.int32)
+    _, vec = np.linalg.eigh(x.T @ x)
+    axis, e1 = vec[:, -1], vec[:, -2]
+    e2 = np.cross(axis, e1)
+    t = x @ axis
+    perp = x - np.outer(t, axis)
+    phi = np.mod(np.arctan2(perp @ e2, perp @ e1), 2.0 * np.pi)
+    el = np.asarray(elements, np.int64)
+    ed = np.concatenate([el[:, [0, 1]], el[:, [1, 2]], el[:, [2, 0]]]) if el.size else np.zeros((0, 2), np.int64)
+    h = float(np.median(np.linalg.norm(x[ed[:, 0]] - x[ed[:, 1]], axis=1))) if ed.size else 1.0
+    h = h if h > 0.0 else 1.0
+    radius = float(np.median(np.linalg.norm(perp, axis=1)))
+    n_ring = max(1.0, 2.0 * np.pi * radius / h)
+    n_strips = max(1, int(round(n_ring / max(1, width))))
+    strip = np.minimum((phi * (n_strips / (2.0 * np.pi))).astype(np.int64), n_strips - 1)
+    # band edges a quarter band off the multiples of h, where the rings of structured tubes sit
+    band = np.floor((t - t.min()) / (0.5 * h) + 0.25).astype(np.int64)
+    return np.lexsort((phi, band, strip)).astype(np.int32)
 
 
 class SparseInfo:

@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from ._lib import CvfesGeometry, CvfesLoad, CvfesSystem, CvfesUpdate, check, ptr
-from .sparse import BlockMaps, bisection_order, block_structure
+from .sparse import BlockMaps, bisection_order, block_structure, strip_order
 
 __all__ = ["MembraneSystem", "LoadTables", "MASS_SCALE"]
 
@@ -123,6 +123,7 @@ class MembraneSystem:
         self.nnzu = maps.nnzu
         self.uvalues = None
         self._nodes_h = nodes
+        self._elems_h = np.asarray(elements, np.int64)
         self.row_order = None
         fixed = np.zeros(self.n_nodes, np.uint8)
         fixed[np.asarray(boundary, np.int64)] = 1
@@ -251,14 +252,23 @@ class MembraneSystem:
             self.assemble_upper()
         self._sys = self._make_struct()
 
-    def set_row_order(self, order="bisection", leaf=64):
-        """Visiting order of the block rows for the wave-uniform stored kernels: ``"natural"``,
-        ``"bisection"`` (recursive coordinate bisection, parts of ``leaf`` nodes) or an explicit
-        permutation.  The result does not depend on it."""
+    def set_row_order(self, order="bisection", leaf=64, width=64):
+        """Visiting order of the block rows for the wave-uniform kernels: ``"natural"``,
+        ``"bisection"`` (recursive coordinate bisection, parts of ``leaf`` nodes), ``"strips"``
+        (angular strips of about ``width`` ring nodes swept along the tube axis; the fastest order
+        for the symmetric-storage kernels) or an explicit permutation.  The result does not depend
+        on it."""
         if order is None or (isinstance(order, str) and order == "natural"):
             self.row_order = None
         else:
-            perm = bisection_order(self._nodes_h, leaf) if isinstance(order, str) else np.asarray(order)
+            if isinstance(order, str) and order == "strips":
+                perm = strip_order(self._nodes_h, self._elems_h, width)
+            elif isinstance(order, str) and order == "bisection":
+                perm = bisection_order(self._nodes_h, leaf)
+            elif isinstance(order, str):
+                raise ValueError(f"unknown row order {order!r}")
+            else:
+                perm = np.asarray(order)
             if perm.shape != (self.n_nodes,) or not np.array_equal(np.sort(perm), np.arange(self.n_nodes)):
                 raise ValueError("row order must be a permutation of the nodes")
             self.row_order = _dev(perm.astype(np.int32), torch.int32, self.device)

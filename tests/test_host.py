@@ -232,3 +232,29 @@ def test_abi_argument_errors_without_gpu():
     assert b"block structure" in lib.cvfes_last_error()
     assert lib.cvfes_element_matrices(None, 3, None, None, 0.3, 5 / 6, None, None, None, None) == -1
     assert lib.cvfes_element_matrices(None, 0, None, None, 0.3, 5 / 6, None, None, None, None) == 0
+
+
+@pytest.mark.parametrize("tilt", [False, True])
+def test_strip_order_is_a_local_permutation(tilt):
+    """The strip visiting order is a permutation, and a node's ring neighbour (same angle, next
+    ring) sits about one strip width back instead of a whole ring, also for a tube whose axis is
+    not a coordinate axis."""
+    from cvfes_amd.sparse import bisection_order, strip_order
+    nth, nz = 256, 120
+    nodes, el, _ = tube_geometry(nth, nz, 0.5, 6.0 * nz / 400)
+    if tilt:
+        a, b = 0.7, -0.4
+        ra = np.array([[1, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
+        rb = np.array([[math.cos(b), 0, math.sin(b)], [0, 1, 0], [-math.sin(b), 0, math.cos(b)]])
+        nodes = nodes @ (rb @ ra).T + np.array([3.0, -1.0, 2.0])
+    N = nodes.shape[0]
+    for perm in (strip_order(nodes, el, 64), bisection_order(nodes, 64)):
+        assert perm.dtype == np.int32 and np.array_equal(np.sort(perm), np.arange(N))
+    perm = strip_order(nodes, el, 64)
+    pos = np.empty(N, np.int64)
+    pos[perm] = np.arange(N)
+    ring = np.abs(pos[nth:] - pos[:-nth])           # node (z, t) and (z + 1, t)
+    assert np.median(ring) <= 2 * 64 and np.percentile(ring, 90) <= 3 * 64
+    same = np.abs(pos[1:] - pos[:-1])[np.arange(N - 1) % nth != nth - 1]   # (z, t) and (z, t + 1)
+    assert np.percentile(same, 90) <= 2
+    assert strip_order(nodes[:0], el[:0], 64).size == 0
