@@ -148,48 +148,97 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
     }
 }
 
-// grad[e,s] += -coef[e,s] * sum_{a,b} nu_a^T khat_e(a,b) x_b,  nu = nu_scale * mu / m,
-// x = xc + bdt (xc - xp) (xp may be NULL).  One thread per (element, sample).
-__global__ __launch_bounds__(kBlock) void k_grad_accum(
-    int64_t n_elms, int64_t n_smp, const int32_t *__restrict__ elems, const int32_t *__restrict__ cpos,
-    const double *__restrict__ khat, const double *__restrict__ coef, const double *__restrict__ lm,
-    int64_t lm_sstride, const double *__restrict__ mu, const double *__restrict__ xc,
-    const double *__restrict__ xp, double bdt, double nu_scale, double *__restrict__ grad) {
-    const int64_t g = xcd_block() * kBlock + threadIdx.x;
-    if (g >= n_elms * n_smp) return;
-    const int64_t e = g / n_smp;
-    const int64_t s = g - e * n_smp;
+struct GradArgs {
+    int64_t n_elms, n_smp;
+    const int32_t *elems, *cpos;
+    const double *khat, *coef, *lm;
+    int64_t lm_sstride;
+    const double *mu, *xc, *xp;
+    double bdt, nu_scale;
+    double *grad;
+};
+
+// sum_{a,b} nu_a^T khat_e(a,b) x_b for the thread's (element e, sample s), with nu = nu_scale * mu / m
+// and x = xc + bdt (xc - xp) (xp may be NULL).  `kh(ab)` gives the 9 entries of unit block ab.
+template <class KH>
+__device__ __forceinline__ double grad_sum(const GradArgs &a, int64_t e, int64_t s, KH kh) {
     int64_t nd[3];
     double nu[9], x[9];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        nd[a] = elems[3 * e + a];
-        const double m = lm[lm_sstride ? nd[a] * n_smp + s : nd[a]];
-        const double sc = m != 0.0 ? nu_scale / m : 0.0;
+    for (int q = 0; q < 3; ++q) {
+        nd[q] = a.elems[3 * e + q];
+        const double m = a.lm[a.lm_sstride ? nd[q] * a.n_smp + s : nd[q]];
+        const double sc = m != 0.0 ? a.nu_scale / m : 0.0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int64_t off = (nd[a] * 3 + j) * n_smp + s;
-            nu[3 * a + j] = sc * mu[off];
-            const double c = xc[off];
-            x[3 * a + j] = xp ? fma(bdt, c - xp[off], c) : c;
+            const int64_t off = (nd[q] * 3 + j) * a.n_smp + s;
+            nu[3 * q + j] = sc * a.mu[off];
+            const double c = a.xc[off];
+            x[3 * q + j] = a.xp ? fma(a.bdt, c - a.xp[off], c) : c;
         }
     }
     double acc = 0.0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int p = 0; p < 3; ++p) {
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const double *kh = khat + 9 * static_cast<int64_t>(cpos[9 * e + 3 * a + b]);
+            const auto *h = kh(3 * p + b);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 double t = 0.0;
 #pragma unroll
-                for (int j = 0; j < 3; ++j) t = fma(kh[3 * i + j], x[3 * b + j], t);
-                acc = fma(nu[3 * a + i], t, acc);
+                for (int j = 0; j < 3; ++j) t = fma(h[3 * i + j], x[3 * b + j], t);
+                acc = fma(nu[3 * p + i], t, acc);
             }
         }
     }
-    grad[g] -= coef[g] * acc;
+    return acc;
+}
+
+// grad[e,s] += -coef[e,s] * grad_sum, one thread per (element, sample), unit blocks read from
+// memory.  UNI (sample count a multiple of 64): every wave holds one element, so its node ids and
+// unit blocks are scalar loads.  At most 80 registers (6 waves per SIMD or more): left alone, the
+// compiler hoists all 81 unit-block loads and runs at 3 waves per SIMD.
+template <bool UNI>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad_accum(GradArgs a) {
+    const int64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g >= a.n_elms * a.n_smp) return;
+    int64_t e = g / a.n_smp;
+    if constexpr (UNI) e = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(e));
+    const int64_t s = g - e * a.n_smp;
+    const double acc = grad_sum(a, e, s, [&](int ab) {
+        return a.khat + 9 * static_cast<int64_t>(a.cpos[9 * e + ab]);
+    });
+    a.grad[g] -= a.coef[g] * acc;
+}
+
+// The same with the unit blocks of the workgroup's elements staged in LDS first (cooperative
+// loads, about 3 per thread at 32 samples), for sample counts that put several elements in a
+// wave: the lanes of one element then read its 81 entries from LDS instead of issuing 81 vector
+// loads each.  Needs n_smp >= kGradLdsMinSmp (at most kGradLdsElems elements per workgroup).
+constexpr int kGradLdsMinSmp = 8;
+constexpr int kGradLdsElems = (kBlock - 1) / kGradLdsMinSmp + 2;   // 33
+
+__global__ __launch_bounds__(kBlock) void k_grad_accum_lds(GradArgs a) {
+    __shared__ double sk[kGradLdsElems * 81];
+    const int64_t g0 = xcd_block() * kBlock;
+    const int64_t total = a.n_elms * a.n_smp;
+    const int64_t e0 = g0 / a.n_smp;
+    int64_t e1 = (g0 + kBlock - 1) / a.n_smp;
+    if (e1 > a.n_elms - 1) e1 = a.n_elms - 1;
+    const int nel = e0 <= e1 ? static_cast<int>(e1 - e0 + 1) : 0;    // <= kGradLdsElems
+    for (int q = threadIdx.x; q < nel * 81; q += kBlock) {
+        const int le = q / 81, r = q - le * 81, ab = r / 9, ij = r - ab * 9;
+        sk[q] = a.khat[9 * static_cast<int64_t>(a.cpos[9 * (e0 + le) + ab]) + ij];
+    }
+    __syncthreads();
+    const int64_t g = g0 + threadIdx.x;
+    if (g >= total) return;
+    const int64_t e = g / a.n_smp;
+    const int64_t s = g - e * a.n_smp;
+    const double *ske = sk + (e - e0) * 81;
+    const double acc = grad_sum(a, e, s, [&](int ab) { return ske + 9 * ab; });
+    a.grad[g] -= a.coef[g] * acc;
 }
 
 template <bool MF, bool UNI, int VW>
@@ -279,9 +328,15 @@ extern "C" int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const i
     CVFES_REQUIRE(elems && cpos && sys->khat && sys->coef && sys->lm && mu && x_cur && grad,
                   "cvfes_grad_accum: null pointer");
     CVFES_REQUIRE(dt > 0.0, "cvfes_grad_accum: dt must be positive");
-    hipLaunchKernelGGL(k_grad_accum, dim3(grid_for(sys->n_elms * sys->n_smp)), dim3(kBlock), 0,
-                       static_cast<hipStream_t>(stream), sys->n_elms, sys->n_smp, elems, cpos,
-                       sys->khat, sys->coef, sys->lm, sys->lm_sstride, mu, x_cur, x_prev,
-                       beta / dt, nu_scale, grad);
+    const GradArgs a{sys->n_elms, sys->n_smp, elems, cpos, sys->khat, sys->coef, sys->lm,
+                     sys->lm_sstride, mu, x_cur, x_prev, beta / dt, nu_scale, grad};
+    const dim3 grid(grid_for(sys->n_elms * sys->n_smp)), block(kBlock);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (sys->n_smp % 64 == 0)
+        hipLaunchKernelGGL(k_grad_accum<true>, grid, block, 0, st, a);
+    else if (sys->n_smp >= kGradLdsMinSmp)
+        hipLaunchKernelGGL(k_grad_accum_lds, grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL(k_grad_accum<false>, grid, block, 0, st, a);
     return check_launch("k_grad_accum");
 }

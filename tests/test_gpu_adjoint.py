@@ -2,7 +2,9 @@
 
 * small tube: the GPU adjoint equals torch autograd through the same recurrence (1e-9 relative),
   for both operators, with Rayleigh beta, damping, a non-zero initial displacement (gradient
-  through u_{-1}) and a checkpoint interval that does not divide the step count;
+  through u_{-1}) and a checkpoint interval that does not divide the step count; sample counts
+  4, 100 and 64 cover the three gradient-accumulation kernels (unit blocks read per lane, staged
+  in LDS, read as scalars);
 * the gradient does not depend on the checkpoint scheme (one or two levels, observations on the
   device or in host memory; bit-exact: recomputation replays the same arithmetic);
 * 49,152-node tube, 32 samples, 2,000 steps, 20 observation times, checkpoints every 100 steps:
@@ -46,6 +48,7 @@ def sine(n, dt):
     ("stored", 2e-7, 0.0, 13, True, 4),
     ("matfree", 0.0, 25.0, 1, True, 4),
     ("stored", 1e-7, 10.0, 20, False, 100),
+    ("matfree", 1e-7, 0.0, 9, True, 64),
 ])
 def test_adjoint_matches_torch_autograd(require_gpu, method, beta, damp, C, u0, S):
     from cvfes_amd.adjoint import AdjointProblem, EnsembleLoss
