@@ -416,6 +416,15 @@ def run_adjoint(args):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     fwd, adj = t1 - t0, t2 - t1
+    fused_steps = prob.stats.get("fused_steps", 0)
+    # the same reverse sweep with recomputation and adjoint steps in separate kernels
+    prob.fuse = False
+    prob.forward()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    g_sep = prob.backward()
+    torch.cuda.synchronize()
+    adj_sep = time.perf_counter() - t3
     out = {
         "metric": "adjoint gradient time / forward solve time (49k-node tube, 32 samples, 2,000 steps)",
         "value": adj / fwd, "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": nsteps,
@@ -426,6 +435,9 @@ def run_adjoint(args):
         "forward_s": fwd, "adjoint_s": adj,
         "forward_sample_steps_per_s": S * nsteps / fwd,
         "gradient_max_abs": float(g.abs().max()),
+        "fused_steps": fused_steps,
+        "adjoint_separate_kernels_s": adj_sep, "ratio_separate_kernels": adj_sep / fwd,
+        "fused_equals_separate": bool(torch.equal(g, g_sep)),
     }
     print(json.dumps(out), flush=True)
 

@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -101,6 +101,10 @@ _SIGNATURES = {
                                   _vp, _f64, _f64, _f64, _int]),
     "cvfes_grad_accum": (_int, [_vp, ctypes.POINTER(CvfesSystem), _vp, _vp, _vp, _vp, _vp, _f64,
                                 _f64, _f64, _vp]),
+    "cvfes_step_kernel": (_i64, [ctypes.POINTER(CvfesSystem), _int]),
+    "cvfes_adjoint_fused_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, ctypes.POINTER(_vp),
+                                        ctypes.POINTER(CvfesLoad), _vp, _vp, _vp, _vp, _vp, _vp,
+                                        _f64, _f64, _f64]),
     "cvfes_element_stress": (_int, [_vp, _i64, _i64, ctypes.POINTER(CvfesGeometry), _vp, _vp, _vp,
                                     _vp]),
     "cvfes_step_updated": (_int, [_vp, ctypes.POINTER(CvfesSystem), ctypes.POINTER(CvfesGeometry),

@@ -26,7 +26,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import check, ptr
+from ._lib import VARIANT_GENERIC, check, ptr
 from .system import LoadTables, MembraneSystem
 
 __all__ = ["AdjointProblem", "EnsembleLoss"]
@@ -45,11 +45,14 @@ class AdjointProblem:
         in (preferably pinned) host memory.
     checkpoint_every : checkpoint interval ``C`` (1 = keep every state, no recomputation).
     inner_every : second-level interval ``C2`` inside a segment (``None`` = one level).
+    fuse : with one level, recompute the segment before the current one during the current one's
+        reverse sweep, each recomputed step in the same kernel as an adjoint step (the operator is
+        read once for both; needs the generic forward kernel and ``C - 1`` more states of memory).
     """
 
     def __init__(self, system: MembraneSystem, load: LoadTables, nsteps, dt, beta=0.0, damp=0.0,
                  method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None,
-                 inner_every=None):
+                 inner_every=None, fuse=True):
         self.sys = system
         self.load = load
         self.nsteps = int(nsteps)
@@ -62,6 +65,7 @@ class AdjointProblem:
         self.u_obs = u_obs
         self.C = max(1, int(checkpoint_every))
         self.C2 = None if inner_every is None or int(inner_every) >= self.C else max(1, int(inner_every))
+        self.fuse = bool(fuse)
         self._obs_dev = None
         N = system.n_nodes
         self.u0 = np.zeros(3 * N) if u0 is None else np.asarray(u0, np.float64)
@@ -131,6 +135,24 @@ class AdjointProblem:
                                           ptr(obs_u), ptr(obs_ref), ptr(w), self.dt, self.beta,
                                           self.damp, project))
 
+    def _fused_call(self, u, up, out, n, mu1, mu2, mu_out, obs_u=None, obs_ref=None, w=None):
+        """Forward step n (u, up -> out) and one projected adjoint step in one kernel."""
+        sysm = self.sys
+        arr = (ctypes.c_void_p * 3)(u.data_ptr(), up.data_ptr(), out.data_ptr())
+        ls = self.load.struct(n)
+        check(sysm.lib.cvfes_adjoint_fused_step(sysm.stream, ctypes.byref(sysm._sys),
+                                                sysm.method_id(self.method), arr, ctypes.byref(ls),
+                                                ptr(mu1), ptr(mu2), ptr(mu_out), ptr(obs_u),
+                                                ptr(obs_ref), ptr(w), self.dt, self.beta, self.damp))
+
+    def fused(self):
+        """Whether the reverse sweep pairs recomputation and adjoint steps in one kernel."""
+        if not self.fuse or self.C2 is not None or self.C < 2 or len(self.checkpoints) < 2:
+            return False
+        sysm = self.sys
+        kern = sysm.lib.cvfes_step_kernel(ctypes.byref(sysm._sys), sysm.method_id(self.method))
+        return kern == VARIANT_GENERIC
+
     def _grad_call(self, mu, xc, xp, nu_scale, grad):
         sysm = self.sys
         check(sysm.lib.cvfes_grad_accum(sysm.stream, ctypes.byref(sysm._sys), ptr(sysm.elems),
@@ -158,15 +180,19 @@ class AdjointProblem:
         self._seg_bufs = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(leaf - 1)]
         self._tmp = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)] \
             if self.C2 is not None else []
-        for s0 in sorted(self.checkpoints, reverse=True):
-            s1 = min(s0 + self.C, N)
-            if self.C2 is None:
-                self._reverse_leaf(s0, s1, self.checkpoints[s0])
-            else:
-                pairs = self._recompute_pairs(s0, s1, self.checkpoints[s0], self.C2)
-                for t0 in sorted(pairs, reverse=True):
-                    self._reverse_leaf(t0, min(t0 + self.C2, s1), pairs[t0])
-                del pairs
+        self.stats["fused_steps"] = 0
+        if self.fused():
+            self._reverse_pipelined(shape)
+        else:
+            for s0 in sorted(self.checkpoints, reverse=True):
+                s1 = min(s0 + self.C, N)
+                if self.C2 is None:
+                    self._reverse_leaf(s0, s1, self.checkpoints[s0])
+                else:
+                    pairs = self._recompute_pairs(s0, s1, self.checkpoints[s0], self.C2)
+                    for t0 in sorted(pairs, reverse=True):
+                        self._reverse_leaf(t0, min(t0 + self.C2, s1), pairs[t0])
+                    del pairs
         mu_n1 = self._mu[0]
         self._seg_bufs = self._tmp = None
         sysm.state = None
@@ -193,6 +219,68 @@ class AdjointProblem:
             if (n + 1 - s0) % C2 == 0:
                 pairs[n + 1] = (cur.clone(), prev.clone())
         return pairs
+
+    def _recompute(self, s0, s1, pair, bufs, upto=None):
+        """States u_{s0-1} .. u_{s1-1} from the pair (u_{s0}, u_{s0-1}) by plain steps into
+        ``bufs``; returns {n: state}.  ``upto`` stops after producing u_{upto} (default s1 - 1)."""
+        sysm = self.sys
+        u_s0, u_sm1 = pair
+        states = {s0 - 1: u_sm1, s0: u_s0}
+        last = s1 - 1 if upto is None else upto
+        for n in range(s0, last):
+            sysm.state = [states[n], states[n - 1], bufs[n - s0]]
+            self._advance(n, 1)
+            states[n + 1] = bufs[n - s0]
+        return states
+
+    def _reverse_pipelined(self, shape):
+        """One-level reverse sweep in which the segment before the current one is recomputed
+        during the current one's adjoint steps, a recomputed step and an adjoint step per fused
+        kernel; the arithmetic of both is that of the separate kernels."""
+        sysm = self.sys
+        dev = sysm.device
+        N = self.nsteps
+        starts = sorted(self.checkpoints, reverse=True)
+        bufs = [self._seg_bufs,
+                [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]]
+        cur = self._recompute(starts[0], min(starts[0] + self.C, N), self.checkpoints[starts[0]], bufs[0])
+        mu_n1, mu_n2, spare = self._mu
+        fused = 0
+        for idx, s0 in enumerate(starts):
+            s1 = min(s0 + self.C, N)
+            nxt = starts[idx + 1] if idx + 1 < len(starts) else None
+            if nxt is not None:
+                nbuf = bufs[(idx + 1) % 2]
+                u_c, up_c = self.checkpoints[nxt]
+                nstates = {nxt - 1: up_c, nxt: u_c}
+                fn, fend = nxt, min(nxt + self.C, N) - 1        # forward steps fn .. fend - 1
+            for n in range(s1 - 1, s0 - 1, -1):
+                self._grad_call(mu_n1, cur[n], cur[n - 1] if self.beta != 0.0 else None,
+                                self.dt * self.dt, self._grad)
+                if n == 0:
+                    break
+                obs_u = cur[n] if n in self.obs_index else None
+                ref = self._obs(n) if obs_u is not None else None
+                w = self._w if obs_u is not None else None
+                if nxt is not None and fn < fend:
+                    out = nbuf[fn - nxt]
+                    self._fused_call(nstates[fn], nstates[fn - 1], out, fn, mu_n1, mu_n2, spare,
+                                     obs_u, ref, w)
+                    nstates[fn + 1] = out
+                    fn += 1
+                    fused += 1
+                else:
+                    self._adjoint_call(mu_n1, mu_n2, spare, obs_u, ref, w)
+                mu_n2, mu_n1, spare = mu_n1, spare, mu_n2
+            if nxt is not None:
+                while fn < fend:                                 # the rest of the recomputation
+                    sysm.state = [nstates[fn], nstates[fn - 1], nbuf[fn - nxt]]
+                    self._advance(fn, 1)
+                    nstates[fn + 1] = nbuf[fn - nxt]
+                    fn += 1
+                cur = nstates
+        self._mu = [mu_n1, mu_n2, spare]
+        self.stats["fused_steps"] = fused
 
     def _reverse_leaf(self, s0, s1, pair):
         """Recompute u_{s0+1} .. u_{s1-1} from (u_{s0}, u_{s0-1}), then run the adjoint and gradient

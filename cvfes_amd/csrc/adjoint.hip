@@ -12,17 +12,13 @@
 // u_{-1} depends on K through the initial acceleration; its adjoint
 //   lambda_{-1} = -(1-gamma) mu_1 + b0 K A mu_1   (u_{-1} is not projected)
 // adds -coef_e (dt^2/2 M^-1 lambda_{-1})_e^T khat_e (u0 + beta du0)_e.
-#include "common.h"
+#include "rowops.h"
 
 namespace cvfes {
 namespace {
 
 struct AdjArgs {
-    int64_t n_nodes, n_smp;
-    const int32_t *indptr, *indices;
-    const double *values;
-    const int32_t *cptr, *celem;
-    const double *khat, *coef;
+    RowArgs k;             // structure and operator (stored values or element data)
     const double *mu1;     // mu_{n+1}
     const double *mu2;     // mu_{n+2}
     double *out;           // mu_n (projected) or lambda_{-1} (unprojected)
@@ -36,44 +32,11 @@ struct AdjArgs {
     int project;
 };
 
-// acc[i][v] += K_p (i, :) . x for block p (K symmetric: the same block serves the transposed
-// product).  Matrix-free: acc += coef_e (khat_c x) over the block's element contributions, without
-// forming the block.
-template <bool MATFREE, int VW>
-__device__ __forceinline__ void adj_block_apply(const AdjArgs &a, int32_t p, int64_t s,
-                                                const double (&x)[3][VW], double (&acc)[3][VW]) {
-    if constexpr (!MATFREE) {
-        const double *base = a.values + (static_cast<int64_t>(p) * 9) * a.n_smp + s;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const Vec<VW> kv = load_vec<VW>(base + (3 * i + j) * a.n_smp);
-#pragma unroll
-                for (int v = 0; v < VW; ++v) acc[i][v] = fma(kv.v[v], x[j][v], acc[i][v]);
-            }
-    } else {
-        for (int32_t c = a.cptr[p]; c < a.cptr[p + 1]; ++c) {
-            const Vec<VW> cf = load_vec<VW>(a.coef + static_cast<int64_t>(a.celem[c]) * a.n_smp + s);
-            const double *kh = a.khat + 9 * static_cast<int64_t>(c);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const double h0 = kh[3 * i], h1 = kh[3 * i + 1], h2 = kh[3 * i + 2];
-#pragma unroll
-                for (int v = 0; v < VW; ++v) {
-                    const double t = fma(h2, x[2][v], fma(h1, x[1][v], h0 * x[0][v]));
-                    acc[i][v] = fma(cf.v[v], t, acc[i][v]);
-                }
-            }
-        }
-    }
-}
-
 template <int VW>
 __device__ __forceinline__ Vec<VW> mass_at(const AdjArgs &a, int64_t node, int64_t s) {
     Vec<VW> m;
     if (a.lm_sstride) {
-        m = load_vec<VW>(a.lm + node * a.n_smp + s);
+        m = load_vec<VW>(a.lm + node * a.k.n_smp + s);
     } else {
         const double mv = a.lm[node];
 #pragma unroll
@@ -82,48 +45,33 @@ __device__ __forceinline__ Vec<VW> mass_at(const AdjArgs &a, int64_t node, int64
     return m;
 }
 
-template <bool UNIFORM, int VW>
-__host__ __device__ __forceinline__ int64_t adj_lanes_per_row(int64_t n_smp) {
-    const int64_t l = n_smp / VW;
-    return UNIFORM ? (l + 63) / 64 * 64 : l;
+// Column term of the adjoint product at node col: x = dt^2/m_col (b1 mu1 - b0 mu2) (0 where the
+// mass is 0); K is symmetric, so the transposed product is the same gathered row product.
+template <bool BETA, int VW>
+__device__ __forceinline__ void adj_column(const AdjArgs &a, int64_t col, int64_t s, double (&x)[3][VW]) {
+    const Vec<VW> mc = mass_at<VW>(a, col, s);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int64_t off = (col * 3 + j) * a.k.n_smp + s;
+        const Vec<VW> m1 = load_vec<VW>(a.mu1 + off);
+        Vec<VW> m2;
+        if constexpr (BETA) m2 = load_vec<VW>(a.mu2 + off);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+            double comb = m1.v[v];
+            if constexpr (BETA) comb = fma(a.bdt, m1.v[v] - m2.v[v], m1.v[v]);
+            x[j][v] = mc.v[v] != 0.0 ? comb * (a.dt2 / mc.v[v]) : 0.0;
+        }
+    }
 }
 
-template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
-__global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
-    const int64_t lpr = adj_lanes_per_row<UNIFORM, VW>(a.n_smp);
-    const int64_t g = xcd_block() * kBlock + threadIdx.x;
-    int64_t r = g / lpr;
-    int64_t s = (g - r * lpr) * VW;
-    const bool active = s < a.n_smp;
-    if (!active) s = a.n_smp - VW;
-    if constexpr (UNIFORM) r = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(r));
-    if (r >= a.n_nodes) return;
-    const int32_t row = static_cast<int32_t>(r);
-    const int64_t S = a.n_smp;
-    double acc[3][VW];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) acc[i][v] = 0.0;
-    for (int32_t p = a.indptr[row]; p < a.indptr[row + 1]; ++p) {
-        const int64_t col = a.indices[p];
-        const Vec<VW> mc = mass_at<VW>(a, col, s);
-        double x[3][VW];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int64_t off = (col * 3 + j) * S + s;
-            const Vec<VW> m1 = load_vec<VW>(a.mu1 + off);
-            Vec<VW> m2;
-            if constexpr (BETA) m2 = load_vec<VW>(a.mu2 + off);
-#pragma unroll
-            for (int v = 0; v < VW; ++v) {
-                double comb = m1.v[v];
-                if constexpr (BETA) comb = fma(a.bdt, m1.v[v] - m2.v[v], m1.v[v]);
-                x[j][v] = mc.v[v] != 0.0 ? comb * (a.dt2 / mc.v[v]) : 0.0;
-            }
-        }
-        adj_block_apply<MATFREE, VW>(a, p, s, x, acc);
-    }
+// lambda_n of one block row from acc = K A (b1 mu1 - b0 mu2) and the observation term; stores
+// mu_n = P lambda_n (or lambda when not projecting) when `active`.  Explicit fma throughout, so
+// the rounding is the same in the separate and the fused kernels.
+template <bool DAMP, int VW>
+__device__ __forceinline__ void adj_finish(const AdjArgs &a, int32_t row, int64_t s, bool active,
+                                           const double (&acc)[3][VW]) {
+    const int64_t S = a.k.n_smp;
     const bool fixed = a.project && a.fixed != nullptr && a.fixed[row] != 0;
     const Vec<VW> m = mass_at<VW>(a, row, s);
 #pragma unroll
@@ -136,16 +84,65 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
         for (int v = 0; v < VW; ++v) {
             double gam = 0.0;
             if constexpr (DAMP) gam = m.v[v] != 0.0 ? a.dtdamp / m.v[v] : 0.0;
-            double lam = (2.0 - gam) * m1.v[v] - (1.0 - gam) * m2.v[v] - acc[i][v];
+            double lam = fma(2.0 - gam, m1.v[v], -fma(1.0 - gam, m2.v[v], acc[i][v]));
             if (a.obs_u) {
                 const double ref = a.obs_ref ? a.obs_ref[off + v] : 0.0;
                 const double w = a.obs_w ? a.obs_w[s + v] : 1.0;
-                lam += w * (a.obs_u[off + v] - ref);
+                lam = fma(w, a.obs_u[off + v] - ref, lam);
             }
             o.v[v] = fixed ? 0.0 : lam;
         }
         if (active) store_vec<VW>(a.out + off, o);
     }
+}
+
+template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
+__global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
+    int32_t row;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    double acc[3][VW];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) acc[i][v] = 0.0;
+    for (int32_t p = a.k.indptr[row]; p < a.k.indptr[row + 1]; ++p) {
+        const int64_t col = a.k.indices[p];
+        double x[3][VW];
+        adj_column<BETA, VW>(a, col, s, x);
+        block_apply<MATFREE, VW>(a.k, p, s, x, acc);
+    }
+    adj_finish<DAMP, VW>(a, row, s, active, acc);
+}
+
+// One forward step (the generic row kernel's operations) and one adjoint step in a single pass
+// over the operator: each block (stored values, or element coefficients and unit blocks) is
+// loaded once and applied to both the forward column u + bdt (u - up) and the adjoint column
+// dt^2/m (b1 mu1 - b0 mu2).  The products and updates are the operations of k_step and
+// k_adjoint_step in the same order, so both outputs equal those of the separate launches bit for
+// bit.  The forward and the adjoint may be at different time steps (the reverse sweep of one
+// checkpoint segment runs with the recomputation of the segment before it).
+template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
+__global__ __launch_bounds__(kBlock) void k_fused_step(StepArgs f, AdjArgs a, int64_t step) {
+    int32_t row;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<UNIFORM, VW>(f.k.n_nodes, f.k.n_smp, row, s, active)) return;
+    double accf[3][VW], acca[3][VW];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) accf[i][v] = acca[i][v] = 0.0;
+    for (int32_t p = f.k.indptr[row]; p < f.k.indptr[row + 1]; ++p) {
+        const int64_t col = f.k.indices[p];
+        double xf[3][VW], xa[3][VW];
+        state_column<BETA, VW>(f.u, f.up, f.bdt, col, f.k.n_smp, s, xf);
+        adj_column<BETA, VW>(a, col, s, xa);
+        block_apply2<MATFREE, VW>(f.k, p, s, xf, accf, xa, acca);
+    }
+    step_update<DAMP, VW>(f, row, s, active, step, accf);
+    adj_finish<DAMP, VW>(a, row, s, active, acca);
 }
 
 struct GradArgs {
@@ -243,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_accum_lds(GradArgs a) {
 
 template <bool MF, bool UNI, int VW>
 void adj_launch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
-    const dim3 g(grid_for(a.n_nodes * adj_lanes_per_row<UNI, VW>(a.n_smp))), b(kBlock);
+    const dim3 g(grid_for(a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp))), b(kBlock);
     if (beta) {
         if (damp) hipLaunchKernelGGL((k_adjoint_step<MF, UNI, true, true, VW>), g, b, 0, st, a);
         else hipLaunchKernelGGL((k_adjoint_step<MF, UNI, true, false, VW>), g, b, 0, st, a);
@@ -253,52 +250,63 @@ void adj_launch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
     }
 }
 
-template <int VW>
-bool adj_uniform(int64_t n_smp) {
-    const int64_t l = n_smp / VW, padded = (l + 63) / 64 * 64;
-    return 4 * l >= 3 * padded;
-}
-
+// Lane width and row mapping as the generic forward kernel chooses them (two samples per lane
+// for even counts; wave-uniform rows when they keep 3/4 of the lanes busy).
 template <bool MF>
 void adj_dispatch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
-    if (a.n_smp % 2 == 0) {
-        if (adj_uniform<2>(a.n_smp)) adj_launch<MF, true, 2>(st, a, beta, damp);
+    if (a.k.n_smp % 2 == 0) {
+        if (use_uniform<2>(a.k.n_smp)) adj_launch<MF, true, 2>(st, a, beta, damp);
         else adj_launch<MF, false, 2>(st, a, beta, damp);
     } else {
-        if (adj_uniform<1>(a.n_smp)) adj_launch<MF, true, 1>(st, a, beta, damp);
+        if (use_uniform<1>(a.k.n_smp)) adj_launch<MF, true, 1>(st, a, beta, damp);
         else adj_launch<MF, false, 1>(st, a, beta, damp);
     }
 }
 
-}  // namespace
-}  // namespace cvfes
+template <bool MF, bool UNI, int VW>
+void fused_launch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta, bool damp) {
+    const dim3 g(grid_for(f.k.n_nodes * lanes_per_row<UNI, VW>(f.k.n_smp))), b(kBlock);
+    if (beta) {
+        if (damp) hipLaunchKernelGGL((k_fused_step<MF, UNI, true, true, VW>), g, b, 0, st, f, a, int64_t(0));
+        else hipLaunchKernelGGL((k_fused_step<MF, UNI, true, false, VW>), g, b, 0, st, f, a, int64_t(0));
+    } else {
+        if (damp) hipLaunchKernelGGL((k_fused_step<MF, UNI, false, true, VW>), g, b, 0, st, f, a, int64_t(0));
+        else hipLaunchKernelGGL((k_fused_step<MF, UNI, false, false, VW>), g, b, 0, st, f, a, int64_t(0));
+    }
+}
 
-using namespace cvfes;
+template <bool MF>
+void fused_dispatch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta, bool damp) {
+    if (f.k.n_smp % 2 == 0) {
+        if (use_uniform<2>(f.k.n_smp)) fused_launch<MF, true, 2>(st, f, a, beta, damp);
+        else fused_launch<MF, false, 2>(st, f, a, beta, damp);
+    } else {
+        if (use_uniform<1>(f.k.n_smp)) fused_launch<MF, true, 1>(st, f, a, beta, damp);
+        else fused_launch<MF, false, 1>(st, f, a, beta, damp);
+    }
+}
 
-extern "C" int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int method,
-                                  const double *mu1, const double *mu2, double *mu_out,
-                                  const double *obs_u, const double *obs_ref, const double *obs_w,
-                                  double dt, double beta, double damp, int project) {
+int check_adjoint_args(const cvfes_system_t *sys, int method, const double *mu1, const double *mu2,
+                       const double *mu_out, double dt, const char *who) {
     CVFES_REQUIRE(sys && sys->indptr && sys->indices && sys->n_nodes >= 1 && sys->n_smp >= 1,
-                  "cvfes_adjoint_step: empty system");
+                  "%s: empty system", who);
     CVFES_REQUIRE(method == CVFES_METHOD_STORED || method == CVFES_METHOD_MATFREE,
-                  "cvfes_adjoint_step: unknown method %d", method);
-    CVFES_REQUIRE(method == CVFES_METHOD_MATFREE || sys->values, "cvfes_adjoint_step: no values");
+                  "%s: unknown method %d", who, method);
+    CVFES_REQUIRE(method == CVFES_METHOD_MATFREE || sys->values, "%s: no values", who);
     CVFES_REQUIRE(method == CVFES_METHOD_STORED || (sys->cptr && sys->celem && sys->khat && sys->coef),
-                  "cvfes_adjoint_step: matrix-free needs cptr/celem/khat/coef");
-    CVFES_REQUIRE(mu1 && mu2 && mu_out && sys->lm, "cvfes_adjoint_step: null pointer");
-    CVFES_REQUIRE(mu_out != mu1 && mu_out != mu2, "cvfes_adjoint_step: output aliases an input");
-    CVFES_REQUIRE(dt > 0.0, "cvfes_adjoint_step: dt must be positive");
+                  "%s: matrix-free needs cptr/celem/khat/coef", who);
+    CVFES_REQUIRE(mu1 && mu2 && mu_out && sys->lm, "%s: null pointer", who);
+    CVFES_REQUIRE(mu_out != mu1 && mu_out != mu2, "%s: output aliases an input", who);
+    CVFES_REQUIRE(dt > 0.0, "%s: dt must be positive", who);
+    return CVFES_OK;
+}
+
+AdjArgs adjoint_args(const cvfes_system_t *sys, const double *mu1, const double *mu2, double *mu_out,
+                     const double *obs_u, const double *obs_ref, const double *obs_w, double dt,
+                     double beta, double damp, int project) {
     AdjArgs a{};
-    a.n_nodes = sys->n_nodes;
-    a.n_smp = sys->n_smp;
-    a.indptr = sys->indptr;
-    a.indices = sys->indices;
-    a.values = sys->values;
-    a.cptr = sys->cptr;
-    a.celem = sys->celem;
-    a.khat = sys->khat;
-    a.coef = sys->coef;
+    a.k = RowArgs{sys->n_nodes, sys->n_smp, sys->indptr, sys->indices, sys->values,
+                  sys->cptr, sys->celem, sys->khat, sys->coef};
     a.mu1 = mu1;
     a.mu2 = mu2;
     a.out = mu_out;
@@ -312,12 +320,72 @@ extern "C" int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int m
     a.bdt = beta / dt;
     a.dtdamp = dt * damp;
     a.project = project;
+    return a;
+}
+
+}  // namespace
+}  // namespace cvfes
+
+using namespace cvfes;
+
+extern "C" int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int method,
+                                  const double *mu1, const double *mu2, double *mu_out,
+                                  const double *obs_u, const double *obs_ref, const double *obs_w,
+                                  double dt, double beta, double damp, int project) {
+    if (int rc = check_adjoint_args(sys, method, mu1, mu2, mu_out, dt, "cvfes_adjoint_step")) return rc;
+    const AdjArgs a = adjoint_args(sys, mu1, mu2, mu_out, obs_u, obs_ref, obs_w, dt, beta, damp, project);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (method == CVFES_METHOD_MATFREE)
         adj_dispatch<true>(st, a, beta != 0.0, damp != 0.0);
     else
         adj_dispatch<false>(st, a, beta != 0.0, damp != 0.0);
     return check_launch("k_adjoint_step");
+}
+
+extern "C" int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys, int method,
+                                        double *const *state, const cvfes_load_t *load,
+                                        const double *mu1, const double *mu2, double *mu_out,
+                                        const double *obs_u, const double *obs_ref,
+                                        const double *obs_w, double dt, double beta, double damp) {
+    if (int rc = check_adjoint_args(sys, method, mu1, mu2, mu_out, dt, "cvfes_adjoint_fused_step"))
+        return rc;
+    CVFES_REQUIRE(state && state[0] && state[1] && state[2], "cvfes_adjoint_fused_step: null state buffer");
+    const double *bufs[6] = {state[0], state[1], state[2], mu1, mu2, mu_out};
+    for (int i = 0; i < 6; ++i)
+        for (int j = i + 1; j < 6; ++j)
+            CVFES_REQUIRE(bufs[i] != bufs[j] || (i == 3 && j == 4),
+                          "cvfes_adjoint_fused_step: state and adjoint buffers must be distinct");
+    CVFES_REQUIRE(sys->f1 && load && load->p && (!load->ga || load->qa) && (!load->gb || load->qb),
+                  "cvfes_adjoint_fused_step: load tables");
+    CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_adjoint_fused_step: lm_sstride");
+    const int64_t kern = cvfes_step_kernel(sys, method);
+    CVFES_REQUIRE(kern == CVFES_VARIANT_GENERIC,
+                  "cvfes_adjoint_fused_step: the forward step of this system runs kernel variant "
+                  "%lld, not the generic one the fused kernel replays",
+                  static_cast<long long>(kern));
+    StepArgs f{};
+    f.k = RowArgs{sys->n_nodes, sys->n_smp, sys->indptr, sys->indices, sys->values,
+                  sys->cptr, sys->celem, sys->khat, sys->coef};
+    f.u = state[0];
+    f.up = state[1];
+    f.out = state[2];
+    f.lm = sys->lm;
+    f.lm_sstride = sys->lm_sstride;
+    f.f1 = sys->f1;
+    f.fixed = sys->fixed;
+    f.load = *load;
+    f.bdy_u = sys->bdy_u;
+    f.dt2 = dt * dt;
+    f.bdt = beta / dt;
+    f.ddt = damp / dt;
+    f.order = nullptr;
+    const AdjArgs a = adjoint_args(sys, mu1, mu2, mu_out, obs_u, obs_ref, obs_w, dt, beta, damp, 1);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (method == CVFES_METHOD_MATFREE)
+        fused_dispatch<true>(st, f, a, beta != 0.0, damp != 0.0);
+    else
+        fused_dispatch<false>(st, f, a, beta != 0.0, damp != 0.0);
+    return check_launch("k_fused_step");
 }
 
 extern "C" int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const int32_t *elems,

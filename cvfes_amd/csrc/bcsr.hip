@@ -502,21 +502,50 @@ int rowload_slots(const cvfes_system_t *sys) {
     return 0;
 }
 
-void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const StepArgs &a,
-                   int64_t step, bool beta, bool damp) {
+// The kernel cvfes_step launches for this system and operator: AUTO, and the variants that the
+// structure, the sample count or the operator does not support, resolved to what actually runs.
+int64_t resolve_variant(const cvfes_system_t *sys, bool mf) {
     const int maxb = rowload_slots(sys);
+    const int64_t S = sys->n_smp;
     int64_t v = sys->variant;
+    const int64_t fallback = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
     // Measured on MI355X (102,400-node tube, 128 samples): the stored operator is fastest with the
     // row-load kernel and non-temporal value loads; the matrix-free operator with the generic
     // kernel (the row-load form needs too many registers for enough waves in flight).
-    if (v == CVFES_VARIANT_AUTO) v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
-    if (mf && (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2) && lds_slots(sys)) {
-        const bool two = v == CVFES_VARIANT_LDS2 && a.k.n_smp % 2 == 0;
-        launch_lds(st, sys, a, step, beta, damp, two);
-        return;
+    if (v == CVFES_VARIANT_AUTO) v = fallback;
+    if (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2) {
+        // LDS staging only exists for the matrix-free operator (and structures within its limits)
+        if (mf && lds_slots(sys))
+            return (v == CVFES_VARIANT_LDS2 && S % 2 == 0) ? CVFES_VARIANT_LDS2 : CVFES_VARIANT_LDS;
+        v = fallback;
     }
-    if (mf && (v == CVFES_VARIANT_CACHED || v == CVFES_VARIANT_CACHED1) && lds_slots(sys) &&
-        use_uniform<2>(a.k.n_smp) && a.k.n_smp % 2 == 0) {
+    if (v == CVFES_VARIANT_CACHED || v == CVFES_VARIANT_CACHED1) {
+        if (mf && lds_slots(sys) && use_uniform<2>(S) && S % 2 == 0) return v;
+        v = fallback;
+    }
+    if (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT || v == CVFES_VARIANT_SYM_NTL) {
+        if (!mf && maxb && sys->uvalues && sys->vblk) return v;
+        v = fallback;
+    }
+    if (maxb && v != CVFES_VARIANT_GENERIC) {
+        const bool one = v == CVFES_VARIANT_ROWLOAD1 || v == CVFES_VARIANT_ROWLOAD1_NT;
+        if (mf) return one ? CVFES_VARIANT_ROWLOAD1 : CVFES_VARIANT_ROWLOAD;   // no non-temporal form
+        return v;
+    }
+    return CVFES_VARIANT_GENERIC;
+}
+
+void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const StepArgs &a,
+                   int64_t step, bool beta, bool damp) {
+    const int maxb = rowload_slots(sys);
+    const int64_t v = resolve_variant(sys, mf);
+    switch (v) {
+    case CVFES_VARIANT_LDS:
+    case CVFES_VARIANT_LDS2:
+        launch_lds(st, sys, a, step, beta, damp, v == CVFES_VARIANT_LDS2);
+        return;
+    case CVFES_VARIANT_CACHED:
+    case CVFES_VARIANT_CACHED1: {
         const LdsArgs l{sys->nptr, sys->nent, sys->cslot};
         const bool six = lds_slots(sys) == 1;
         if (v == CVFES_VARIANT_CACHED) {
@@ -528,10 +557,9 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
         }
         return;
     }
-    if (v == CVFES_VARIANT_CACHED || v == CVFES_VARIANT_CACHED1)
-        v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
-    const bool sym = v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT || v == CVFES_VARIANT_SYM_NTL;
-    if (!mf && sym && maxb && sys->uvalues && sys->vblk) {
+    case CVFES_VARIANT_SYM:
+    case CVFES_VARIANT_SYM_NT:
+    case CVFES_VARIANT_SYM_NTL: {
         const int nt = v == CVFES_VARIANT_SYM ? 0 : (v == CVFES_VARIANT_SYM_NT ? 1 : 2);
         if (maxb == 7) {
             if (nt == 0) launch_sym<0, 7>(st, sys, a, step, beta, damp);
@@ -544,11 +572,10 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
         }
         return;
     }
-    if (sym) v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
-    // LDS staging only exists for the matrix-free operator (and structures within its limits)
-    if (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2)
-        v = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
-    if (maxb && v != CVFES_VARIANT_GENERIC) {
+    case CVFES_VARIANT_ROWLOAD:
+    case CVFES_VARIANT_ROWLOAD_NT:
+    case CVFES_VARIANT_ROWLOAD1:
+    case CVFES_VARIANT_ROWLOAD1_NT: {
         const bool one = v == CVFES_VARIANT_ROWLOAD1 || v == CVFES_VARIANT_ROWLOAD1_NT;
         const bool nt = v == CVFES_VARIANT_ROWLOAD_NT || v == CVFES_VARIANT_ROWLOAD1_NT;
         if (mf) {
@@ -563,6 +590,9 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
         }
         return;
     }
+    default:
+        break;
+    }
     const bool v2 = a.k.n_smp % 2 == 0;
     if (mf) {
         if (v2) step_u<true, 2>(st, a, step, beta, damp); else step_u<true, 1>(st, a, step, beta, damp);
@@ -575,6 +605,12 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
 }  // namespace cvfes
 
 using namespace cvfes;
+
+extern "C" int64_t cvfes_step_kernel(const cvfes_system_t *sys, int method) {
+    if (!sys || (method != CVFES_METHOD_STORED && method != CVFES_METHOD_MATFREE)) return -1;
+    if (sys->variant < CVFES_VARIANT_AUTO || sys->variant > CVFES_VARIANT_CACHED1) return -1;
+    return resolve_variant(sys, method == CVFES_METHOD_MATFREE);
+}
 
 extern "C" int cvfes_assemble_bcsr(void *stream, int64_t nnzb, int64_t n_smp, const int32_t *cptr,
                                    const int32_t *celem, const double *khat, const double *coef,

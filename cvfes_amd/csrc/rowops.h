@@ -150,6 +150,64 @@ __device__ __forceinline__ void block_apply(const RA &k, int32_t p, int64_t s,
     }
 }
 
+// Column term of the step's product at node col: x = u + bdt (u - up) when BETA, else u.
+template <bool BETA, int VW, class DP>
+__device__ __forceinline__ void state_column(DP u, DP up, double bdt, int64_t col, int64_t n_smp,
+                                             int64_t s, double (&x)[3][VW]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int64_t off = (col * 3 + j) * n_smp + s;
+        const Vec<VW> xu = load_vec<VW>(u + off);
+        if constexpr (BETA) {
+            const Vec<VW> xp = load_vec<VW>(up + off);
+#pragma unroll
+            for (int v = 0; v < VW; ++v) x[j][v] = fma(bdt, xu.v[v] - xp.v[v], xu.v[v]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VW; ++v) x[j][v] = xu.v[v];
+        }
+    }
+}
+
+// block_apply for two columns at once (the block is loaded once): acc += K_p x and acc2 += K_p x2,
+// each with exactly the operations block_apply performs.
+template <bool MATFREE, int VW, class RA>
+__device__ __forceinline__ void block_apply2(const RA &k, int32_t p, int64_t s,
+                                             const double (&x)[3][VW], double (&acc)[3][VW],
+                                             const double (&x2)[3][VW], double (&acc2)[3][VW]) {
+    if constexpr (!MATFREE) {
+        double kv[9][VW];
+        stored_block<VW>(k, p, s, kv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int v = 0; v < VW; ++v) {
+                    acc[i][v] = fma(kv[3 * i + j][v], x[j][v], acc[i][v]);
+                    acc2[i][v] = fma(kv[3 * i + j][v], x2[j][v], acc2[i][v]);
+                }
+    } else {
+        const int32_t c0 = k.cptr[p], c1 = k.cptr[p + 1];
+        for (int32_t c = c0; c < c1; ++c) {
+            const int64_t e = k.celem[c];
+            const Vec<VW> cf = load_vec<VW>(k.coef + e * k.n_smp + s);
+            const auto *kh = k.khat + 9 * static_cast<int64_t>(c);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double h0 = kh[3 * i], h1 = kh[3 * i + 1], h2 = kh[3 * i + 2];
+#pragma unroll
+                for (int v = 0; v < VW; ++v) {
+                    const double t = fma(h2, x[2][v], fma(h1, x[1][v], h0 * x[0][v]));
+                    acc[i][v] = fma(cf.v[v], t, acc[i][v]);
+                    const double t2 = fma(h2, x2[2][v], fma(h1, x2[1][v], h0 * x2[0][v]));
+                    acc2[i][v] = fma(cf.v[v], t2, acc2[i][v]);
+                }
+            }
+        }
+    }
+}
+
 // acc[i][v] = sum over the row's blocks of K_p (i, :) . x_col, with x = u + bdt (u - up) when BETA.
 template <bool MATFREE, bool BETA, int VW, class RA, class DP>
 __device__ __forceinline__ void row_product(const RA &k, DP u, DP up, double bdt, int32_t row,
@@ -162,30 +220,18 @@ __device__ __forceinline__ void row_product(const RA &k, DP u, DP up, double bdt
     for (int32_t p = p0; p < p1; ++p) {
         const int64_t col = k.indices[p];
         double x[3][VW];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int64_t off = (col * 3 + j) * k.n_smp + s;
-            const Vec<VW> xu = load_vec<VW>(u + off);
-            if constexpr (BETA) {
-                const Vec<VW> xp = load_vec<VW>(up + off);
-#pragma unroll
-                for (int v = 0; v < VW; ++v) x[j][v] = fma(bdt, xu.v[v] - xp.v[v], xu.v[v]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < VW; ++v) x[j][v] = xu.v[v];
-            }
-        }
+        state_column<BETA, VW>(u, up, bdt, col, k.n_smp, s, x);
         block_apply<MATFREE, VW>(k, p, s, x, acc);
     }
 }
 
-// Block-row product, central-difference update, load and Dirichlet value for one block row and
-// VW consecutive samples; stores u_{n+1} when `active`.
-template <bool MATFREE, bool BETA, bool DAMP, int VW, class SA>
-__device__ __forceinline__ void step_row(const SA &a, int32_t row, int64_t s, bool active,
-                                         int64_t step) {
-    double acc[3][VW];
-    row_product<MATFREE, BETA, VW>(a.k, a.u, a.up, a.bdt, row, s, acc);
+// Central-difference update of one block row and VW consecutive samples from its product
+// acc = K x: load, damping, lumped mass and Dirichlet value; stores u_{n+1} when `active`.  Every
+// product and sum that could be contracted is an explicit fma, so the rounding is the same in
+// every kernel that uses it (single-mesh, batched, and the step fused with the adjoint).
+template <bool DAMP, int VW, class SA>
+__device__ __forceinline__ void step_update(const SA &a, int32_t row, int64_t s, bool active,
+                                            int64_t step, const double (&acc)[3][VW]) {
     const double pn = a.load.p[step];
     const double qa = a.load.ga ? a.load.qa[step] : 0.0;
     const double qb = a.load.gb ? a.load.qb[step] : 0.0;
@@ -199,12 +245,11 @@ __device__ __forceinline__ void step_row(const SA &a, int32_t row, int64_t s, bo
         const Vec<VW> pp = load_vec<VW>(a.up + off);
         const int64_t d = 3 * static_cast<int64_t>(row) + i;
         double fi = pn * a.f1[d];
-        if (a.load.ga) fi += qa * a.load.ga[d];
-        if (a.load.gb) fi += qb * a.load.gb[d];
+        if (a.load.ga) fi = fma(qa, a.load.ga[d], fi);
+        if (a.load.gb) fi = fma(qb, a.load.gb[d], fi);
         Vec<VW> o;
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
-            // explicit fused operations: the same rounding for every lane width
             const double m = a.lm_sstride ? lmp[v] : lmp[0];
             double force = fi;
             if constexpr (DAMP) force = fma(-a.ddt, uu.v[v] - pp.v[v], force);
@@ -214,6 +259,15 @@ __device__ __forceinline__ void step_row(const SA &a, int32_t row, int64_t s, bo
         }
         if (active) store_vec<VW>(a.out + off, o);
     }
+}
+
+// Block-row product and central-difference update of one block row and VW consecutive samples.
+template <bool MATFREE, bool BETA, bool DAMP, int VW, class SA>
+__device__ __forceinline__ void step_row(const SA &a, int32_t row, int64_t s, bool active,
+                                         int64_t step) {
+    double acc[3][VW];
+    row_product<MATFREE, BETA, VW>(a.k, a.u, a.up, a.bdt, row, s, acc);
+    step_update<DAMP, VW>(a, row, s, active, step, acc);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -37,6 +37,20 @@ from . import vtkio
 __all__ = ["GPUSolidSolver"]
 
 
+def update_interval(config):
+    """Re-formation interval of the updated-geometry mode, or None for the linear mode.
+
+    The reference re-forms K(X + u) when its step counter reaches ``stiffness_update_interval``;
+    an interval no smaller than the run's step count (``len(np.arange(endtime, step=dt))``,
+    solver.py:48) never does, which is the linear mode (the shipped cfgs say 1000000000)."""
+    interval = getattr(config, "update_interval", None)
+    if interval is None:
+        return None
+    endtime = getattr(config, "endtime", None)
+    n_run = len(np.arange(float(endtime), step=float(config.dt))) if endtime is not None else 1 << 40
+    return int(interval) if int(interval) < n_run else None
+
+
 class GPUSolidSolver(PhysicsSolver):
     """Ensemble membrane solid step on one MI355X (one process per GPU)."""
 
@@ -73,8 +87,7 @@ class GPUSolidSolver(PhysicsSolver):
         self.damp = float(mesh.damp) if self.damped else 0.0
         self.beta = float(getattr(config, "beta", 0.0) or 0.0)
         self.method = getattr(config, "stiffness_method", "stored") or "stored"
-        interval = getattr(config, "update_interval", None)
-        self.updateInterval = int(interval) if interval is not None and int(interval) < (1 << 40) else None
+        self.updateInterval = update_interval(config)
         self.massLumping = getattr(config, "mass_lumping", "cpu") or "cpu"
         self.time0 = float(getattr(config, "time", 0.0) or 0.0)
 

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 6
+#define CVFES_ABI_VERSION 7
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -212,6 +212,13 @@ int cvfes_step(void *stream, const cvfes_system_t *sys, int method, double **sta
                const cvfes_load_t *load, int64_t nsteps, double dt, double beta, double damp);
 
 /*
+ * The step kernel cvfes_step launches for this system and method (CVFES_VARIANT_*, with AUTO and
+ * variants the structure or sample count does not support resolved), or -1 for bad arguments.
+ * Host only.  No counterpart in the reference (one kernel there).
+ */
+int64_t cvfes_step_kernel(const cvfes_system_t *sys, int method);
+
+/*
  * One step of the adjoint recurrence of cvfes_step (no counterpart in the reference, which has no
  * gradient; it is the transpose of the step above, K being symmetric):
  *   lambda = (2-gamma) mu1 - (1-gamma) mu2 - K A (mu1 + (beta/dt)(mu1 - mu2)) + w (obs_u - obs_ref)
@@ -223,6 +230,22 @@ int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int method, cons
                        const double *mu2, double *mu_out, const double *obs_u,
                        const double *obs_ref, const double *obs_w, double dt, double beta,
                        double damp, int project);
+
+/*
+ * One forward step (as cvfes_step with nsteps = 1: state = {u_n, u_{n-1}, out}, load entry 0 of
+ * `load`) and one projected adjoint step (as cvfes_adjoint_step with project = 1) in one kernel
+ * that reads the operator once for both; the two may be at different time steps.  Both outputs
+ * equal those of the separate calls bit for bit.  Needs cvfes_step_kernel(sys, method) to be
+ * CVFES_VARIANT_GENERIC (the kernel whose operations it replays) and six distinct buffers
+ * (mu1 may equal mu2).  The state array is not rotated.  No counterpart in the reference (used by
+ * the checkpointed reverse sweep: the recomputation of one segment runs with the adjoint steps of
+ * the next).
+ */
+int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys, int method,
+                             double *const *state, const cvfes_load_t *load, const double *mu1,
+                             const double *mu2, double *mu_out, const double *obs_u,
+                             const double *obs_ref, const double *obs_w, double dt, double beta,
+                             double damp);
 
 /*
  * Gradient with respect to theta = log E per element and sample (no counterpart in the reference):

@@ -6,7 +6,8 @@
   4, 100 and 64 cover the three gradient-accumulation kernels (unit blocks read per lane, staged
   in LDS, read as scalars);
 * the gradient does not depend on the checkpoint scheme (one or two levels, observations on the
-  device or in host memory; bit-exact: recomputation replays the same arithmetic);
+  device or in host memory, recomputation fused with the adjoint steps or not; bit-exact:
+  recomputation replays the same arithmetic);
 * 49,152-node tube, 32 samples, 2,000 steps, 20 observation times, checkpoints every 100 steps:
   central finite differences with step 1e-6 on 20 seeded (element, sample) pairs, each within
   1e-6 relative of the adjoint.  Two separate 2,000-step FP64 runs at theta +- h would carry a
@@ -81,6 +82,33 @@ def test_adjoint_matches_torch_autograd(require_gpu, method, beta, damp, C, u0, 
     g = thd.grad.cpu().numpy()
     assert np.max(np.abs(gref)) > 0
     assert np.max(np.abs(g - gref)) / np.max(np.abs(gref)) < 1e-9
+
+
+@pytest.mark.parametrize("method,S", [("stored", 8), ("matfree", 8), ("stored", 5), ("matfree", 66)])
+def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
+    """The pipelined sweep (recomputation of a segment fused with the adjoint steps of the next)
+    gives the gradient of the separate kernels bit for bit, with beta, damping, an initial
+    displacement, observations on both kinds of step and a checkpoint interval that does not
+    divide the step count; and it does fuse steps."""
+    from cvfes_amd.adjoint import AdjointProblem
+    nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 7)
+    nsteps, dt = 75, 4e-6
+    p = sine(nsteps, dt)
+    obs = [8, 30, 31, 64, 75]
+    rng = np.random.default_rng(2)
+    u_obs = torch.as_tensor(1e-5 * rng.standard_normal((len(obs), nodes.shape[0], 3, S))).cuda()
+    u0 = 1e-6 * rng.standard_normal(3 * nodes.shape[0])
+    grads = []
+    for fuse in (False, True):
+        prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0, method=method,
+                              obs_steps=obs, u_obs=u_obs, checkpoint_every=11, u0=u0, fuse=fuse)
+        prob.set_theta(torch.as_tensor(theta).cuda())
+        prob.forward()
+        grads.append(prob.backward().cpu().numpy())
+        assert prob.fused() == fuse
+        assert (prob.stats["fused_steps"] > 50) == fuse
+    assert np.max(np.abs(grads[0])) > 0
+    assert np.array_equal(grads[0], grads[1])
 
 
 def test_gradient_independent_of_checkpoint_interval(require_gpu):

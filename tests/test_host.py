@@ -234,6 +234,51 @@ def test_abi_argument_errors_without_gpu():
     assert lib.cvfes_element_matrices(None, 0, None, None, 0.3, 5 / 6, None, None, None, None) == 0
 
 
+def test_step_kernel_resolution_without_gpu():
+    """cvfes_step_kernel (host only) names the kernel cvfes_step would launch: AUTO and the
+    variants a structure, sample count or operator does not support fall back as documented."""
+    import ctypes
+    from cvfes_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libcvfes_mi355x.so not built")
+    lib = _lib.lib()
+    V = _lib.VARIANTS
+    dummy = ctypes.c_void_p(16)                 # never dereferenced: resolution reads the struct only
+
+    def kern(S, variant, method, uvalues=False, lds=False, limits=(7, 2, 6)):
+        sysd = _lib.CvfesSystem(n_nodes=4, n_smp=S, nnzb=4, variant=V[variant],
+                                max_row_blocks=limits[0], max_offdiag_contrib=limits[1],
+                                max_diag_contrib=limits[2])
+        if uvalues:
+            sysd.uvalues = sysd.vblk = dummy
+        if lds:
+            sysd.nptr = sysd.nent = sysd.cslot = dummy
+            sysd.max_node_elems = 6
+        return lib.cvfes_step_kernel(ctypes.byref(sysd), _lib.METHODS[method])
+
+    assert kern(128, "auto", "stored") == V["rowload_nt"]
+    assert kern(128, "auto", "matfree") == V["generic"]
+    assert kern(100, "auto", "stored") == V["generic"]          # row-load needs S % 128 == 0
+    assert kern(128, "auto", "stored", limits=(7, 3, 6)) == V["generic"]
+    assert kern(128, "rowload_nt", "matfree") == V["rowload"]   # no non-temporal matrix-free form
+    assert kern(128, "rowload1_nt", "matfree") == V["rowload1"]
+    assert kern(128, "sym_ntl", "stored") == V["rowload_nt"]    # no symmetric storage formed
+    assert kern(128, "sym_ntl", "stored", uvalues=True) == V["sym_ntl"]
+    assert kern(128, "sym", "matfree", uvalues=True) == V["generic"]
+    assert kern(128, "lds2", "matfree") == V["generic"]         # no node-element lists
+    assert kern(128, "lds2", "matfree", lds=True) == V["lds2"]
+    assert kern(127, "lds2", "matfree", lds=True) == V["lds"]   # odd S: one sample per lane
+    assert kern(128, "lds", "stored", lds=True) == V["rowload_nt"]
+    assert kern(128, "cached", "matfree", lds=True) == V["cached"]
+    assert kern(66, "cached", "matfree", lds=True) == V["generic"]    # 33 of 64 lanes: not uniform
+    assert kern(100, "cached", "matfree", lds=True) == V["cached"]    # 50 of 64 lanes: uniform
+    assert kern(128, "generic", "stored") == V["generic"]
+    assert lib.cvfes_step_kernel(None, 0) == -1
+    assert kern(128, "auto", "stored") == kern(128, "auto", "bcsr")
+    sysd = _lib.CvfesSystem(n_nodes=4, n_smp=8, nnzb=4, variant=99)
+    assert lib.cvfes_step_kernel(ctypes.byref(sysd), 0) == -1
+
+
 @pytest.mark.parametrize("tilt", [False, True])
 def test_strip_order_is_a_local_permutation(tilt):
     """The strip visiting order is a permutation, and a node's ring neighbour (same angle, next
@@ -258,3 +303,21 @@ def test_strip_order_is_a_local_permutation(tilt):
     same = np.abs(pos[1:] - pos[:-1])[np.arange(N - 1) % nth != nth - 1]   # (z, t) and (z, t + 1)
     assert np.percentile(same, 90) <= 2
     assert strip_order(nodes[:0], el[:0], 64).size == 0
+
+
+def test_update_interval_selects_linear_mode_from_the_run_length():
+    """An interval at least the run's step count never re-forms the stiffness: linear mode."""
+    from types import SimpleNamespace as NS
+    from cvfes_amd.physicsSolverSDampGPUs import update_interval
+    dt = 2.0 ** -18
+    run = NS(dt=dt, endtime=200 * dt)                          # 200 steps
+    assert update_interval(NS(**vars(run), update_interval=1)) == 1
+    assert update_interval(NS(**vars(run), update_interval=199)) == 199
+    assert update_interval(NS(**vars(run), update_interval=200)) is None
+    assert update_interval(NS(**vars(run), update_interval=1000000000)) is None
+    assert update_interval(NS(**vars(run), update_interval=None)) is None
+    assert update_interval(NS(dt=dt, update_interval=1 << 62)) is None      # no end time known
+    assert update_interval(NS(dt=dt, update_interval=5)) == 5
+    cfg = CVConfig(read_cfg(os.path.join(os.path.dirname(__file__), "..", "cvfes_amd", "cfg",
+                                         "tube_64x32_s16.cfg")))
+    assert update_interval(cfg.solver) is None                 # the shipped cfg: linear mode
