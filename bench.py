@@ -212,7 +212,8 @@ def main():
         # the other operator and the symmetric storage of the stored one, measured the same way
         alts = [("matfree", "auto", "natural")] if args.method == "stored" else [("stored", "auto", "natural")]
         if args.method == "stored" and args.variant == "auto":
-            alts += [("stored", "sym_ntl", "natural"), ("stored", "sym_ntl", "strips")]
+            alts += [("stored", "sym_ntl", "natural"), ("stored", "sym_ntl", "strips"),
+                     ("matfree", "ebe", "natural")]
         out["alternatives"] = []
         for other, variant, order in alts:
             sysm.set_variant(variant)

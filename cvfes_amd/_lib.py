@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -25,11 +25,12 @@ VARIANT_ROWLOAD1, VARIANT_ROWLOAD1_NT = 4, 5
 VARIANT_LDS, VARIANT_LDS2 = 6, 7
 VARIANT_SYM, VARIANT_SYM_NT, VARIANT_SYM_NTL = 8, 9, 10
 VARIANT_CACHED, VARIANT_CACHED1 = 11, 12
+VARIANT_EBE = 13
 VARIANTS = {"auto": VARIANT_AUTO, "generic": VARIANT_GENERIC, "rowload": VARIANT_ROWLOAD,
             "rowload_nt": VARIANT_ROWLOAD_NT, "rowload1": VARIANT_ROWLOAD1,
             "rowload1_nt": VARIANT_ROWLOAD1_NT, "lds": VARIANT_LDS, "lds2": VARIANT_LDS2,
             "sym": VARIANT_SYM, "sym_nt": VARIANT_SYM_NT, "sym_ntl": VARIANT_SYM_NTL,
-            "cached": VARIANT_CACHED, "cached1": VARIANT_CACHED1}
+            "cached": VARIANT_CACHED, "cached1": VARIANT_CACHED1, "ebe": VARIANT_EBE}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -52,6 +53,8 @@ class CvfesSystem(ctypes.Structure):
         ("max_row_blocks", _i64), ("max_offdiag_contrib", _i64), ("max_diag_contrib", _i64),
         ("variant", _i64), ("nptr", _vp), ("nent", _vp), ("cslot", _vp), ("max_node_elems", _i64),
         ("uvalues", _vp), ("vblk", _vp), ("nnzu", _i64), ("row_order", _vp),
+        ("color_ptr", _vp), ("color_elems", _vp), ("elems", _vp), ("cpos", _vp), ("n_colors", _i64),
+        ("max_color_elems", _i64),
     ]
 
 

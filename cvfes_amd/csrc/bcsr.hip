@@ -11,9 +11,10 @@
 //
 // The stiffness operator is either
 //   METHOD_STORED : values[p][ij][s] read from HBM, or
-//   METHOD_MATFREE: values rebuilt per block as sum_c coef[celem[c], s] * khat[c][ij]
-//                   (sample-independent khat, scalar loads; coef [E][S] is cached across the 3
-//                   rows that touch an element).
+//   METHOD_MATFREE: each block applied as sum_c coef[celem[c], s] * (khat[c] x) over its element
+//                   contributions (sample-independent khat, scalar loads; coef [E][S] is cached
+//                   across the 3 rows that touch an element), or, in the EBE variant, element by
+//                   element over element colours.
 #include "rowops.h"
 
 namespace cvfes {
@@ -502,6 +503,122 @@ int rowload_slots(const cvfes_system_t *sys) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Element-by-element matrix-free variant (EBE): K x as a sum of element products,
+// y_a += coef_e sum_b khat_e(a,b) x_b, one launch per element colour (no two elements of a colour
+// share a node, so the read-modify-writes of y do not race), then the per-row update from y.  y
+// lives in the output buffer: it is zeroed first, and the update reads it and overwrites it row
+// by row.  Every node's y is read and written once per incident element, so this moves several
+// times the bytes of the row gather; it is kept as the measured alternative.
+
+__global__ __launch_bounds__(kBlock) void k_ebe_zero(double *__restrict__ y, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) y[i] = 0.0;
+}
+
+// One thread per (element of colour c, VW samples); lanes map to samples as for block rows.
+template <bool UNIFORM, bool BETA, int VW>
+__global__ __launch_bounds__(kBlock) void k_ebe_colour(StepArgs a, const int32_t *__restrict__ color_ptr,
+                                                      const int32_t *__restrict__ color_elems,
+                                                      const int32_t *__restrict__ elems,
+                                                      const int32_t *__restrict__ cpos, int c) {
+    const int32_t c0 = color_ptr[c], c1 = color_ptr[c + 1];
+    int32_t i;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<UNIFORM, VW>(c1 - c0, a.k.n_smp, i, s, active)) return;
+    const int64_t S = a.k.n_smp;
+    const int64_t e = color_elems[c0 + i];
+    int64_t nd[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) nd[q] = elems[3 * e + q];
+    double x[3][3][VW];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) state_column<BETA, VW>(a.u, a.up, a.bdt, nd[b], S, s, x[b]);
+    const Vec<VW> cf = load_vec<VW>(a.k.coef + e * S + s);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double t[3][VW];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int v = 0; v < VW; ++v) t[r][v] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double *kh = a.k.khat + 9 * static_cast<int64_t>(cpos[9 * e + 3 * q + b]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double h0 = kh[3 * r], h1 = kh[3 * r + 1], h2 = kh[3 * r + 2];
+#pragma unroll
+                for (int v = 0; v < VW; ++v)
+                    t[r][v] = fma(h2, x[b][2][v], fma(h1, x[b][1][v], fma(h0, x[b][0][v], t[r][v])));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int64_t off = (nd[q] * 3 + r) * S + s;
+            Vec<VW> y = load_vec<VW>(a.out + off);
+#pragma unroll
+            for (int v = 0; v < VW; ++v) y.v[v] = fma(cf.v[v], t[r][v], y.v[v]);
+            if (active) store_vec<VW>(a.out + off, y);
+        }
+    }
+}
+
+template <bool UNIFORM, bool DAMP, int VW>
+__global__ __launch_bounds__(kBlock) void k_ebe_finish(StepArgs a, int64_t step) {
+    int32_t row;
+    int64_t s;
+    bool active;
+    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    double acc[3][VW];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const Vec<VW> y = load_vec<VW>(a.out + (static_cast<int64_t>(row) * 3 + i) * a.k.n_smp + s);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) acc[i][v] = y.v[v];
+    }
+    step_update<DAMP, VW>(a, row, s, active, step, acc);
+}
+
+template <bool UNI, int VW>
+void ebe_inst(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step, bool beta,
+              bool damp) {
+    const int64_t S = a.k.n_smp;
+    const int64_t n = a.k.n_nodes * 3 * S;
+    hipLaunchKernelGGL(k_ebe_zero, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, st, a.out, n);
+    const dim3 gc(grid_for(sys->max_color_elems * lanes_per_row<UNI, VW>(S))), b(kBlock);
+    for (int c = 0; c < static_cast<int>(sys->n_colors); ++c) {
+        if (beta)
+            hipLaunchKernelGGL((k_ebe_colour<UNI, true, VW>), gc, b, 0, st, a, sys->color_ptr,
+                               sys->color_elems, sys->elems, sys->cpos, c);
+        else
+            hipLaunchKernelGGL((k_ebe_colour<UNI, false, VW>), gc, b, 0, st, a, sys->color_ptr,
+                               sys->color_elems, sys->elems, sys->cpos, c);
+    }
+    const dim3 gf(grid_for(a.k.n_nodes * lanes_per_row<UNI, VW>(S)));
+    if (damp) hipLaunchKernelGGL((k_ebe_finish<UNI, true, VW>), gf, b, 0, st, a, step);
+    else hipLaunchKernelGGL((k_ebe_finish<UNI, false, VW>), gf, b, 0, st, a, step);
+}
+
+void launch_ebe(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step, bool beta,
+                bool damp) {
+    const int64_t S = a.k.n_smp;
+    if (S % 2 == 0) {
+        if (use_uniform<2>(S)) ebe_inst<true, 2>(st, sys, a, step, beta, damp);
+        else ebe_inst<false, 2>(st, sys, a, step, beta, damp);
+    } else {
+        if (use_uniform<1>(S)) ebe_inst<true, 1>(st, sys, a, step, beta, damp);
+        else ebe_inst<false, 1>(st, sys, a, step, beta, damp);
+    }
+}
+
+bool ebe_ready(const cvfes_system_t *sys) {
+    return sys->color_ptr && sys->color_elems && sys->elems && sys->cpos && sys->n_colors > 0 &&
+           sys->max_color_elems > 0 && sys->max_color_elems <= sys->n_elms;
+}
+
 // The kernel cvfes_step launches for this system and operator: AUTO, and the variants that the
 // structure, the sample count or the operator does not support, resolved to what actually runs.
 int64_t resolve_variant(const cvfes_system_t *sys, bool mf) {
@@ -523,6 +640,10 @@ int64_t resolve_variant(const cvfes_system_t *sys, bool mf) {
         if (mf && lds_slots(sys) && use_uniform<2>(S) && S % 2 == 0) return v;
         v = fallback;
     }
+    if (v == CVFES_VARIANT_EBE) {
+        if (mf && ebe_ready(sys)) return v;
+        v = fallback;
+    }
     if (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT || v == CVFES_VARIANT_SYM_NTL) {
         if (!mf && maxb && sys->uvalues && sys->vblk) return v;
         v = fallback;
@@ -540,6 +661,9 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
     const int maxb = rowload_slots(sys);
     const int64_t v = resolve_variant(sys, mf);
     switch (v) {
+    case CVFES_VARIANT_EBE:
+        launch_ebe(st, sys, a, step, beta, damp);
+        return;
     case CVFES_VARIANT_LDS:
     case CVFES_VARIANT_LDS2:
         launch_lds(st, sys, a, step, beta, damp, v == CVFES_VARIANT_LDS2);
@@ -608,7 +732,7 @@ using namespace cvfes;
 
 extern "C" int64_t cvfes_step_kernel(const cvfes_system_t *sys, int method) {
     if (!sys || (method != CVFES_METHOD_STORED && method != CVFES_METHOD_MATFREE)) return -1;
-    if (sys->variant < CVFES_VARIANT_AUTO || sys->variant > CVFES_VARIANT_CACHED1) return -1;
+    if (sys->variant < CVFES_VARIANT_AUTO || sys->variant > CVFES_VARIANT_EBE) return -1;
     return resolve_variant(sys, method == CVFES_METHOD_MATFREE);
 }
 
@@ -692,7 +816,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     CVFES_REQUIRE(nsteps >= 0, "cvfes_step: nsteps < 0");
     CVFES_REQUIRE(dt > 0.0, "cvfes_step: dt must be positive");
     CVFES_REQUIRE(sys->lm_sstride == 0 || sys->lm_sstride == 1, "cvfes_step: lm_sstride must be 0 or 1");
-    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_CACHED1,
+    CVFES_REQUIRE(sys->variant >= CVFES_VARIANT_AUTO && sys->variant <= CVFES_VARIANT_EBE,
                   "cvfes_step: unknown variant %lld", static_cast<long long>(sys->variant));
     hipStream_t st = static_cast<hipStream_t>(stream);
     StepArgs a{};

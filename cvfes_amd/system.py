@@ -125,6 +125,8 @@ class MembraneSystem:
         self._nodes_h = nodes
         self._elems_h = np.asarray(elements, np.int64)
         self.row_order = None
+        self.color_ptr = self.color_elems = None      # element colouring (EBE variant), on demand
+        self.n_colors = self.max_color_elems = 0
         fixed = np.zeros(self.n_nodes, np.uint8)
         fixed[np.asarray(boundary, np.int64)] = 1
         self.fixed = _dev(fixed, torch.uint8, dev)
@@ -241,15 +243,41 @@ class MembraneSystem:
             max_offdiag_contrib=self.max_offdiag_contrib, max_diag_contrib=self.max_diag_contrib,
             variant=self.variant, nptr=ptr(self.nptr), nent=ptr(self.nent), cslot=ptr(self.cslot),
             max_node_elems=self.max_node_elems, uvalues=ptr(self.uvalues), vblk=ptr(self.vblk),
-            nnzu=self.nnzu, row_order=ptr(self.row_order))
+            nnzu=self.nnzu, row_order=ptr(self.row_order), color_ptr=ptr(self.color_ptr),
+            color_elems=ptr(self.color_elems), elems=ptr(self.elems), cpos=ptr(self.cpos),
+            n_colors=self.n_colors, max_color_elems=self.max_color_elems)
 
     def set_variant(self, variant):
         """Select the step kernel variant (names in ``_lib.VARIANTS``); the symmetric ones form the
-        upper-block storage on first use."""
+        upper-block storage on first use, the element-by-element one the element colouring."""
         self.variant = _lib.VARIANTS[variant] if isinstance(variant, str) else int(variant)
         if self.variant in (_lib.VARIANT_SYM, _lib.VARIANT_SYM_NT, _lib.VARIANT_SYM_NTL) and self.uvalues is None \
                 and self.values is not None:
             self.assemble_upper()
+        if self.variant == _lib.VARIANT_EBE and self.color_ptr is None:
+            self.set_coloring()
+        self._sys = self._make_struct()
+
+    def set_coloring(self, groups=None):
+        """Element colouring for the element-by-element variant: ``groups`` (element-id arrays, no
+        two elements of a group sharing a node; default: the greedy ``mesh_coloring``)."""
+        if groups is None:
+            from .coloring import mesh_coloring
+            groups = mesh_coloring(self._elems_h, self.n_nodes, n_colors=64)[1]
+        groups = [np.asarray(g, np.int64) for g in groups if len(g)]
+        flat = np.concatenate(groups) if groups else np.zeros(0, np.int64)
+        if not np.array_equal(np.sort(flat), np.arange(self.n_elms)):
+            raise ValueError("the colour groups must partition the elements")
+        for g in groups:
+            nodes = self._elems_h[g].ravel()
+            if np.unique(nodes).size != nodes.size:
+                raise ValueError("two elements of one colour share a node")
+        ptrs = np.zeros(len(groups) + 1, np.int64)
+        np.cumsum([len(g) for g in groups], out=ptrs[1:])
+        self.color_ptr = _dev(ptrs.astype(np.int32), torch.int32, self.device)
+        self.color_elems = _dev(flat.astype(np.int32), torch.int32, self.device)
+        self.n_colors = len(groups)
+        self.max_color_elems = max((len(g) for g in groups), default=0)
         self._sys = self._make_struct()
 
     def set_row_order(self, order="bisection", leaf=64, width=64):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 7
+#define CVFES_ABI_VERSION 8
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -78,6 +78,15 @@ typedef struct cvfes_system {
      * 0..N-1; NULL = natural).  Any permutation gives the same result; a compact one (neighbouring
      * nodes close together) lets the second read of a mirrored block hit in L2. */
     const int32_t *row_order;
+    /* Element colouring for the element-by-element variant (EBE; may be NULL otherwise): the
+     * elements of colour c are color_elems[color_ptr[c] .. color_ptr[c+1]), and no two of them
+     * share a node (meshColoring.py:7-63 greedy colouring).  elems is [E][3] node ids. */
+    const int32_t *color_ptr;   /* [n_colors+1]                                                  */
+    const int32_t *color_elems; /* [E]                                                           */
+    const int32_t *elems;       /* [E][3]                                                        */
+    const int32_t *cpos;        /* [9E] contribution index of element e's block (a, b)          */
+    int64_t n_colors;
+    int64_t max_color_elems;    /* largest colour (launch bound)                                  */
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */
@@ -96,6 +105,10 @@ typedef struct cvfes_system {
 #define CVFES_VARIANT_CACHED 11    /* matrix-free: the row node's element coefficients loaded once
                                       per row (two samples per lane)                             */
 #define CVFES_VARIANT_CACHED1 12   /* as CACHED with one sample per lane                            */
+#define CVFES_VARIANT_EBE 13       /* matrix-free element by element: one pass per element colour
+                                    * adds each element's K_e x_e into the nodes' sums (no write
+                                    * races inside a colour), then a per-row update (needs the
+                                    * colouring fields)                                             */
 
 /* Time-dependent nodal load: f_n = p[n] * f1 + qa[n] * ga + qb[n] * gb  (ga/gb may be NULL).
  * ga, gb are sample-independent [N][3] traction fields (e.g. two consecutive fluid wall-stress

@@ -146,7 +146,7 @@ def case128(require_gpu):
 
 @pytest.mark.parametrize("variant", ["generic", "rowload", "rowload_nt", "rowload1", "rowload1_nt",
                                      "lds", "lds2", "sym", "sym_nt", "sym_ntl",
-                                     "cached", "cached1"])
+                                     "cached", "cached1", "ebe"])
 @pytest.mark.parametrize("method", ["stored", "matfree"])
 def test_kernel_variants_match_oracle(case128, method, variant):
     """Every step-kernel variant at 128 samples (wave-uniform rows) against the oracle."""
@@ -165,10 +165,13 @@ def test_structure_limits(case128):
 
 
 @pytest.mark.parametrize("n_smp", [97, 100])
-@pytest.mark.parametrize("method", ["stored", "matfree"])
+@pytest.mark.parametrize("method", ["stored", "matfree", "matfree-ebe"])
 def test_padded_wavefront_rows_match_oracle(require_gpu, method, n_smp):
     """Sample counts that are not a multiple of the wavefront (rows padded to whole wavefronts)."""
     c = Case("tube-64x32-s16", n_smp=n_smp, seed=3)
+    if method == "matfree-ebe":
+        method = "matfree"
+        c.sys.set_variant("ebe")
     out, snaps = _run_both(c, 150, method, beta=1e-6, damp=20.0, dt=3e-6, record=(1, 60))
     for k, u in out.items():
         assert rel_err(u, snaps[k]) < REL_U, f"step {k}"
@@ -203,7 +206,7 @@ def test_row_order_does_not_change_result(case128, method, variant):
 @pytest.mark.parametrize("method,variant", [("stored", "generic"), ("stored", "rowload_nt"),
                                             ("stored", "sym_ntl"), ("matfree", "generic"),
                                             ("matfree", "lds"), ("matfree", "cached"),
-                                            ("matfree", "cached1")])
+                                            ("matfree", "cached1"), ("matfree", "ebe")])
 def test_isolated_node_every_kernel(require_gpu, method, variant):
     """A node in no element (empty block row, zero mass) next to a tube: every kernel keeps it at
     zero and matches the oracle elsewhere."""

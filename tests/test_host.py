@@ -245,7 +245,7 @@ def test_step_kernel_resolution_without_gpu():
     V = _lib.VARIANTS
     dummy = ctypes.c_void_p(16)                 # never dereferenced: resolution reads the struct only
 
-    def kern(S, variant, method, uvalues=False, lds=False, limits=(7, 2, 6)):
+    def kern(S, variant, method, uvalues=False, lds=False, limits=(7, 2, 6), colors=False):
         sysd = _lib.CvfesSystem(n_nodes=4, n_smp=S, nnzb=4, variant=V[variant],
                                 max_row_blocks=limits[0], max_offdiag_contrib=limits[1],
                                 max_diag_contrib=limits[2])
@@ -254,6 +254,9 @@ def test_step_kernel_resolution_without_gpu():
         if lds:
             sysd.nptr = sysd.nent = sysd.cslot = dummy
             sysd.max_node_elems = 6
+        if colors:
+            sysd.color_ptr = sysd.color_elems = sysd.elems = sysd.cpos = dummy
+            sysd.n_elms, sysd.n_colors, sysd.max_color_elems = 10, 3, 4
         return lib.cvfes_step_kernel(ctypes.byref(sysd), _lib.METHODS[method])
 
     assert kern(128, "auto", "stored") == V["rowload_nt"]
@@ -273,6 +276,10 @@ def test_step_kernel_resolution_without_gpu():
     assert kern(66, "cached", "matfree", lds=True) == V["generic"]    # 33 of 64 lanes: not uniform
     assert kern(100, "cached", "matfree", lds=True) == V["cached"]    # 50 of 64 lanes: uniform
     assert kern(128, "generic", "stored") == V["generic"]
+    assert kern(128, "ebe", "matfree") == V["generic"]          # no element colouring
+    assert kern(128, "ebe", "matfree", colors=True) == V["ebe"]
+    assert kern(97, "ebe", "matfree", colors=True) == V["ebe"]
+    assert kern(128, "ebe", "stored", colors=True) == V["rowload_nt"]
     assert lib.cvfes_step_kernel(None, 0) == -1
     assert kern(128, "auto", "stored") == kern(128, "auto", "bcsr")
     sysd = _lib.CvfesSystem(n_nodes=4, n_smp=8, nnzb=4, variant=99)
