@@ -48,6 +48,8 @@ def algorithmic_bytes(sysm, method):
     state = 3 * (3 * N * S * 8)                 # read u, read up, write u+
     lm = sysm.lm.numel() * 8
     small = 3 * N * 8 + 4 * (N + 1) + 4 * nnzb + N   # f1, indptr, indices, fixed
+    if getattr(sysm, "row_order", None) is not None:
+        small += 4 * N                                 # the visiting order
     if method == "stored" and getattr(sysm, "variant", 0) in (8, 9, 10):  # symmetric storage
         op = sysm.nnzu * 9 * S * 8 + 4 * nnzb                          # upper values, vblk
     elif method == "stored":
@@ -139,6 +141,9 @@ def main():
     ap.add_argument("--case", default="tube-256x400-s128")
     ap.add_argument("--method", default="stored", choices=["stored", "matfree"])
     ap.add_argument("--variant", default="auto", help="step kernel variant (cvfes_amd._lib.VARIANTS)")
+    ap.add_argument("--row-order", default="auto", choices=["auto", "natural", "strips"],
+                    help="row visiting order of the timed step (auto: the fastest of natural and "
+                         "strips 32/64 on 20 steps each; the result does not depend on it)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="CPU baseline steps (first is warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
     ap.add_argument("--no-cpu", action="store_true")
@@ -179,6 +184,13 @@ def main():
     sysm.set_variant(args.variant)
     total = args.warmup + args.steps
     load = sysm.loads(case.pressures(0, max(total, args.cpu_steps + 1), dt))
+    occ = sysm.autotune_occupancy(load, dt, beta, method=args.method)
+    if args.row_order == "auto":
+        order = sysm.autotune_row_order(load, dt, beta, method=args.method)
+    else:
+        sysm.set_row_order(args.row_order, width=32)
+        order = {"order": args.row_order, "width": 32 if args.row_order == "strips" else 0}
+    order_name = order["order"] + (str(order["width"]) if order["order"] == "strips" else "")
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
     print(f"[bench] setup {t_setup:.1f} s: N={N} E={el.shape[0]} nnzb={sysm.nnzb} S={S} dt={dt}",
@@ -199,12 +211,14 @@ def main():
         "data": "synthetic: seeded tube mesh and per-element GMRF Young's modulus samples",
         "config": {"workload": case.name, "n_nodes": N, "n_elements": int(el.shape[0]),
                    "nnzb": sysm.nnzb, "n_samples": S, "dt": dt, "beta": beta,
-                   "operator": args.method, "variant": args.variant,
+                   "operator": args.method, "variant": args.variant, "row_order": order_name,
+                   "waves_per_simd": occ["waves_per_simd"],
                    "layout": "values [nnzb][9][S], state [N][3][S]"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(case.name, args.method if args.variant == "auto"
-                                                 else f"{args.method}-{args.variant}"),
+                     "traffic": measured_traffic(case.name, (args.method if args.variant == "auto"
+                                                             else f"{args.method}-{args.variant}")
+                                                 + ("" if order_name == "natural" else f"-{order_name}")),
                      "algorithmic_bytes_per_launch": abytes, "kernel_ms": kern_ms,
                      "kernel": "k_step (fused block-CSR product + central-difference update)"},
     }
@@ -215,22 +229,26 @@ def main():
             alts += [("stored", "sym_ntl", "natural"), ("stored", "sym_ntl", "strips"),
                      ("matfree", "ebe", "natural")]
         out["alternatives"] = []
+        main_occ = sysm.waves_per_simd
         for other, variant, order in alts:
             sysm.set_variant(variant)
             sysm.set_row_order(order)
+            aocc = sysm.autotune_occupancy(load, dt, beta, method=other, steps=5)
             msteps = max(20, args.steps // 2)
             mwall, mk = time_steps(sysm, load, dt, beta, other, args.warmup, msteps)
             mbytes = algorithmic_bytes(sysm, other)
             key = other if variant == "auto" else f"{other}-{variant}"
             sysm.set_row_order("natural")
             out["alternatives"].append({
-                "operator": other, "variant": variant, "row_order": order, "value": S * msteps / mwall,
+                "operator": other, "variant": variant, "row_order": order,
+                "waves_per_simd": aocc["waves_per_simd"], "value": S * msteps / mwall,
                 "unit": "sample-steps/s", "kernel_ms": mk, "achieved_GBs": mbytes / (mk * 1e-3) / 1e9,
                 "frac": mbytes / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": mbytes,
                 "traffic": measured_traffic(case.name, key)})
             print(f"[bench] {key} ({order} order): {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
                   file=sys.stderr, flush=True)
         sysm.set_variant(args.variant)
+        sysm.set_occupancy(main_occ)
     if not args.no_cpu:
         from oracle import cpu_oracle as co
         threads = args.cpu_threads or co.max_threads()
@@ -470,12 +488,14 @@ def run_combined(args):
     per_step = {}
     finals = {}
     for method in ("stored", "matfree"):
+        occ = sysm.autotune_occupancy(load, dt, beta, method=method, steps=5)
         wall, kern_ms = time_steps(sysm, load, dt, beta, method, 5, 50)
         per_step[method] = {"kernel_ms": kern_ms, "sample_steps_per_s": S / (wall / 50),
                             "achieved_GBs": algorithmic_bytes(sysm, method) / (kern_ms * 1e-3) / 1e9,
-                            "algorithmic_bytes_per_launch": algorithmic_bytes(sysm, method)}
+                            "algorithmic_bytes_per_launch": algorithmic_bytes(sysm, method),
+                            "waves_per_simd": occ["waves_per_simd"]}
         finals[method] = sysm.u.cpu().numpy()
-        log(f"{method}: {kern_ms:.3f} ms/step")
+        log(f"{method}: {kern_ms:.3f} ms/step at waves_per_simd {occ['waves_per_simd']}")
     agree = float(np.max(np.abs(finals["stored"] - finals["matfree"])) / np.max(np.abs(finals["stored"])))
     del finals
     sysm.release_values()

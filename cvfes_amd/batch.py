@@ -49,6 +49,7 @@ class BatchedEnsemble:
         # rows are wave-uniform with two (packed rows take one)
         self.vw_hint = 0
         self.cap = False          # cap the kernel at 64 registers (8 waves per SIMD, possible spills)
+        self.waves_per_simd = 0   # cap on the waves per SIMD (3..7, unused LDS; 0 = none)
         self.kinds = 0
         self.phase = 0
         self.step_index = 0
@@ -94,7 +95,7 @@ class BatchedEnsemble:
         self._drop_graphs()
 
     def _kinds(self):
-        return self.kinds | (32 if self.cap else 0)
+        return self.kinds | (32 if self.cap else 0) | (self.waves_per_simd << 6)
 
     def _drop_graphs(self):
         for g in self._graphs.values():
@@ -158,8 +159,10 @@ class BatchedEnsemble:
         self.phase = 0
 
     # ------------------------------------------------------------------------------------------
-    def autotune(self, candidates=None, steps=6, lane_widths=(0, 1, 2), caps=(False, True)):
-        """Pick the lane width, register cap and grid size (wavefronts) with the shortest step."""
+    def autotune(self, candidates=None, steps=6, lane_widths=(0, 1, 2), caps=(False, True),
+                 occupancies=(0, 6, 5, 4)):
+        """Pick the lane width, register cap and grid size (wavefronts) with the shortest step,
+        then the cap on the waves per SIMD for that choice."""
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         best = None
         saved = (self.phase, self.step_index)
@@ -172,22 +175,35 @@ class BatchedEnsemble:
                 self.cap = cap
                 for w in grid:
                     self.n_waves = w
-                    self._eager(1)
-                    torch.cuda.synchronize(self.device)
-                    t0 = time.perf_counter()
-                    self._eager(steps)
-                    torch.cuda.synchronize(self.device)
-                    dt = (time.perf_counter() - t0) / steps
+                    dt = self._time(steps)
                     if best is None or dt < best[3]:
-                        best = (vw, cap, w, dt)
+                        best = (vw, cap, w, dt, 0)
         self.vw_hint, self.cap, self.n_waves = best[0], best[1], best[2]
+        self._sync_roles()
+        self._replan()
+        for occ in occupancies:
+            if occ == 0:
+                continue
+            self.waves_per_simd = occ
+            dt = self._time(steps)
+            if dt < best[3]:
+                best = best[:3] + (dt, occ)
+        self.waves_per_simd = best[4]
         # the tuning steps advanced the state: restart the bookkeeping from the current roles
         self._sync_roles()
         self._replan()
         names = {0: "two where even", 1: "one everywhere", 2: "two where rows are uniform"}
         self.tuned = {"lane_width": names[best[0]], "register_cap": best[1], "n_waves": best[2],
-                      "ms_per_step": best[3] * 1e3, "from_step": saved[1]}
+                      "waves_per_simd": best[4], "ms_per_step": best[3] * 1e3, "from_step": saved[1]}
         return self.tuned
+
+    def _time(self, steps):
+        self._eager(1)
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        self._eager(steps)
+        torch.cuda.synchronize(self.device)
+        return (time.perf_counter() - t0) / steps
 
     def _replan(self):
         """Re-plan from the meshes' current buffer roles (keeps the step index)."""

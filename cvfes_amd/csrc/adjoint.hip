@@ -30,6 +30,7 @@ struct AdjArgs {
     const double *obs_w;   // [S] weight of the observation term per sample (NULL = 1)
     double dt2, bdt, dtdamp;
     int project;
+    unsigned occ_lds;      // launch only: unused LDS per workgroup that caps the occupancy
 };
 
 template <int VW>
@@ -242,11 +243,11 @@ template <bool MF, bool UNI, int VW>
 void adj_launch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
     const dim3 g(grid_for(a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp))), b(kBlock);
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_adjoint_step<MF, UNI, true, true, VW>), g, b, 0, st, a);
-        else hipLaunchKernelGGL((k_adjoint_step<MF, UNI, true, false, VW>), g, b, 0, st, a);
+        if (damp) hipLaunchKernelGGL((k_adjoint_step<MF, UNI, true, true, VW>), g, b, a.occ_lds, st, a);
+        else hipLaunchKernelGGL((k_adjoint_step<MF, UNI, true, false, VW>), g, b, a.occ_lds, st, a);
     } else {
-        if (damp) hipLaunchKernelGGL((k_adjoint_step<MF, UNI, false, true, VW>), g, b, 0, st, a);
-        else hipLaunchKernelGGL((k_adjoint_step<MF, UNI, false, false, VW>), g, b, 0, st, a);
+        if (damp) hipLaunchKernelGGL((k_adjoint_step<MF, UNI, false, true, VW>), g, b, a.occ_lds, st, a);
+        else hipLaunchKernelGGL((k_adjoint_step<MF, UNI, false, false, VW>), g, b, a.occ_lds, st, a);
     }
 }
 
@@ -267,11 +268,11 @@ template <bool MF, bool UNI, int VW>
 void fused_launch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta, bool damp) {
     const dim3 g(grid_for(f.k.n_nodes * lanes_per_row<UNI, VW>(f.k.n_smp))), b(kBlock);
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_fused_step<MF, UNI, true, true, VW>), g, b, 0, st, f, a, int64_t(0));
-        else hipLaunchKernelGGL((k_fused_step<MF, UNI, true, false, VW>), g, b, 0, st, f, a, int64_t(0));
+        if (damp) hipLaunchKernelGGL((k_fused_step<MF, UNI, true, true, VW>), g, b, f.occ_lds, st, f, a, int64_t(0));
+        else hipLaunchKernelGGL((k_fused_step<MF, UNI, true, false, VW>), g, b, f.occ_lds, st, f, a, int64_t(0));
     } else {
-        if (damp) hipLaunchKernelGGL((k_fused_step<MF, UNI, false, true, VW>), g, b, 0, st, f, a, int64_t(0));
-        else hipLaunchKernelGGL((k_fused_step<MF, UNI, false, false, VW>), g, b, 0, st, f, a, int64_t(0));
+        if (damp) hipLaunchKernelGGL((k_fused_step<MF, UNI, false, true, VW>), g, b, f.occ_lds, st, f, a, int64_t(0));
+        else hipLaunchKernelGGL((k_fused_step<MF, UNI, false, false, VW>), g, b, f.occ_lds, st, f, a, int64_t(0));
     }
 }
 
@@ -298,6 +299,8 @@ int check_adjoint_args(const cvfes_system_t *sys, int method, const double *mu1,
     CVFES_REQUIRE(mu1 && mu2 && mu_out && sys->lm, "%s: null pointer", who);
     CVFES_REQUIRE(mu_out != mu1 && mu_out != mu2, "%s: output aliases an input", who);
     CVFES_REQUIRE(dt > 0.0, "%s: dt must be positive", who);
+    CVFES_REQUIRE(sys->waves_per_simd == 0 || (sys->waves_per_simd >= 3 && sys->waves_per_simd <= 7),
+                  "%s: waves_per_simd must be 0 or 3..7", who);
     return CVFES_OK;
 }
 
@@ -320,6 +323,7 @@ AdjArgs adjoint_args(const cvfes_system_t *sys, const double *mu1, const double 
     a.bdt = beta / dt;
     a.dtdamp = dt * damp;
     a.project = project;
+    a.occ_lds = occupancy_lds(sys->waves_per_simd);
     return a;
 }
 
@@ -379,6 +383,7 @@ extern "C" int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys,
     f.bdt = beta / dt;
     f.ddt = damp / dt;
     f.order = nullptr;
+    f.occ_lds = occupancy_lds(sys->waves_per_simd);
     const AdjArgs a = adjoint_args(sys, mu1, mu2, mu_out, obs_u, obs_ref, obs_w, dt, beta, damp, 1);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (method == CVFES_METHOD_MATFREE)

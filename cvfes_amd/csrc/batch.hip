@@ -185,6 +185,13 @@ BatchKernel batch_kernel(int64_t kinds, bool one) {
     return cap ? batch_kernel_sb<false, true>(kinds) : batch_kernel_sb<false, false>(kinds);
 }
 
+// kinds from cvfes_batch_plan (bits 0..4, at least one lane path), the caller's register cap (bit
+// 5) and waves-per-SIMD cap (bits 6..8: 0 or 3..7).
+bool valid_kinds(int64_t kinds) {
+    const int64_t w = (kinds >> 6) & 7;
+    return kinds >= 1 && kinds < 512 && (kinds & 19) && (w == 0 || w >= 3);
+}
+
 __global__ void k_counter_add(int64_t *counter, int64_t by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += by;
 }
@@ -281,11 +288,11 @@ extern "C" int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh
                                 int64_t n_waves, int64_t kinds) {
     CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && n_tasks >= 0, "cvfes_batch_step: bad arguments");
     CVFES_REQUIRE(phase >= 0 && phase < 3, "cvfes_batch_step: phase must be 0, 1 or 2");
-    CVFES_REQUIRE(kinds >= 1 && kinds <= 63 && (kinds & 19), "cvfes_batch_step: kinds must come from cvfes_batch_plan");
+    CVFES_REQUIRE(valid_kinds(kinds), "cvfes_batch_step: kinds must come from cvfes_batch_plan");
     if (n_tasks == 0) return CVFES_OK;
     const int64_t w = n_waves > 0 ? n_waves : default_waves();
-    hipLaunchKernelGGL(batch_kernel(kinds, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock), 0,
-                       static_cast<hipStream_t>(stream),
+    hipLaunchKernelGGL(batch_kernel(kinds & 63, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock),
+                       occupancy_lds((kinds >> 6) & 7), static_cast<hipStream_t>(stream),
                        static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks, phase,
                        counter, k_local);
     return check_launch("k_batch_step");
@@ -301,7 +308,7 @@ extern "C" int cvfes_counter_add(void *stream, int64_t *counter, int64_t by) {
 extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int64_t n_tasks,
                                         int64_t *counter, int64_t block_steps, int phase0,
                                         int64_t n_waves, int64_t kinds, void **graph) {
-    CVFES_REQUIRE(kinds >= 1 && kinds <= 63 && (kinds & 19), "cvfes_batch_graph_create: kinds must come from cvfes_batch_plan");
+    CVFES_REQUIRE(valid_kinds(kinds), "cvfes_batch_graph_create: kinds must come from cvfes_batch_plan");
     CVFES_REQUIRE(dev_meshes && n_mesh >= 1 && counter && graph, "cvfes_batch_graph_create: bad arguments");
     CVFES_REQUIRE(block_steps >= 3 && block_steps % 3 == 0,
                   "cvfes_batch_graph_create: block_steps must be a positive multiple of 3");
@@ -312,11 +319,12 @@ extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int6
     if (err != hipSuccess) return fail(CVFES_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(err));
     err = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (err != hipSuccess) {
-        hipStreamDestroy(cs);
+        (void)hipStreamDestroy(cs);
         return fail(CVFES_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(err));
     }
     for (int64_t k = 0; k < block_steps; ++k)
-        hipLaunchKernelGGL(batch_kernel(kinds, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock), 0, cs,
+        hipLaunchKernelGGL(batch_kernel(kinds & 63, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock),
+                           occupancy_lds((kinds >> 6) & 7), cs,
                            static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks,
                            static_cast<int>((phase0 + k) % 3), counter, k);
     hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, cs, counter, block_steps);
@@ -324,8 +332,8 @@ extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int6
     err = hipStreamEndCapture(cs, &g);
     hipGraphExec_t exec = nullptr;
     if (err == hipSuccess) err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    if (g) hipGraphDestroy(g);
-    hipStreamDestroy(cs);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
     if (err != hipSuccess) return fail(CVFES_ERR_HIP, "graph capture: %s", hipGetErrorString(err));
     *graph = exec;
     return CVFES_OK;

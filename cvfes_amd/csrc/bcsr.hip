@@ -342,6 +342,8 @@ int validate(const cvfes_system_t *sys, int method, const char *who) {
     else
         CVFES_REQUIRE(sys->cptr && sys->celem && sys->khat && sys->coef,
                       "%s: matrix-free method needs cptr/celem/khat/coef", who);
+    CVFES_REQUIRE(sys->waves_per_simd == 0 || (sys->waves_per_simd >= 3 && sys->waves_per_simd <= 7),
+                  "%s: waves_per_simd must be 0 or 3..7", who);
     return CVFES_OK;
 }
 
@@ -356,7 +358,7 @@ RowArgs row_args(const cvfes_system_t *sys) {
 template <bool MF, bool UNI, int VW>
 void launch_spmv(hipStream_t st, const StepArgs &a) {
     const int64_t work = a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp);
-    hipLaunchKernelGGL((k_spmv<MF, UNI, VW>), dim3(grid_for(work)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_spmv<MF, UNI, VW>), dim3(grid_for(work)), dim3(kBlock), a.occ_lds, st, a);
 }
 
 template <bool MF, int VW>
@@ -379,7 +381,7 @@ void spmv_dispatch(hipStream_t st, bool mf, const StepArgs &a) {
 template <bool MF, bool UNI, bool BETA, bool DAMP, int VW>
 void launch_step(hipStream_t st, const StepArgs &a, int64_t step) {
     const int64_t work = a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp);
-    hipLaunchKernelGGL((k_step<MF, UNI, BETA, DAMP, VW>), dim3(grid_for(work)), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_step<MF, UNI, BETA, DAMP, VW>), dim3(grid_for(work)), dim3(kBlock), a.occ_lds,
                        st, a, step);
 }
 
@@ -407,11 +409,11 @@ void launch_rowload(hipStream_t st, const StepArgs &a, int64_t step, bool beta, 
     const int64_t work = a.k.n_nodes * lanes_per_row<true, VW>(a.k.n_smp);
     const dim3 g(grid_for(work)), b(kBlock);
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_step_rowload<MF, true, true, NT, MAXB, VW>), g, b, 0, st, a, step);
-        else hipLaunchKernelGGL((k_step_rowload<MF, true, false, NT, MAXB, VW>), g, b, 0, st, a, step);
+        if (damp) hipLaunchKernelGGL((k_step_rowload<MF, true, true, NT, MAXB, VW>), g, b, a.occ_lds, st, a, step);
+        else hipLaunchKernelGGL((k_step_rowload<MF, true, false, NT, MAXB, VW>), g, b, a.occ_lds, st, a, step);
     } else {
-        if (damp) hipLaunchKernelGGL((k_step_rowload<MF, false, true, NT, MAXB, VW>), g, b, 0, st, a, step);
-        else hipLaunchKernelGGL((k_step_rowload<MF, false, false, NT, MAXB, VW>), g, b, 0, st, a, step);
+        if (damp) hipLaunchKernelGGL((k_step_rowload<MF, false, true, NT, MAXB, VW>), g, b, a.occ_lds, st, a, step);
+        else hipLaunchKernelGGL((k_step_rowload<MF, false, false, NT, MAXB, VW>), g, b, a.occ_lds, st, a, step);
     }
 }
 
@@ -448,11 +450,11 @@ void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, in
     const double *uv = sys->uvalues;
     const int32_t *vb = sys->vblk;
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
-        else hipLaunchKernelGGL((k_step_sym<true, false, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
+        else hipLaunchKernelGGL((k_step_sym<true, false, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
     } else {
-        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
-        else hipLaunchKernelGGL((k_step_sym<false, false, NT, MAXB, 2>), grid, block, 0, st, a, uv, vb, step);
+        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
+        else hipLaunchKernelGGL((k_step_sym<false, false, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
     }
 }
 
@@ -768,6 +770,7 @@ extern "C" int cvfes_bcsr_spmv(void *stream, const cvfes_system_t *sys, int meth
     a.k = row_args(sys);
     a.u = x;
     a.out = y;
+    a.occ_lds = occupancy_lds(sys->waves_per_simd);
     spmv_dispatch(static_cast<hipStream_t>(stream), method == CVFES_METHOD_MATFREE, a);
     return check_launch("k_spmv");
 }
@@ -831,6 +834,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     a.bdt = beta / dt;
     a.ddt = damp / dt;
     a.order = sys->row_order;
+    a.occ_lds = occupancy_lds(sys->waves_per_simd);
     const bool mf = method == CVFES_METHOD_MATFREE;
     double *cur = state[0], *prev = state[1], *nxt = state[2];
     for (int64_t n = 0; n < nsteps; ++n) {

@@ -49,6 +49,14 @@ inline unsigned grid_for(int64_t work) {
     return static_cast<unsigned>(blocks);
 }
 
+// Unused LDS per 256-thread workgroup (one wave on each SIMD) that caps a kernel at w waves per
+// SIMD: a CU's 160 KiB then holds w such workgroups and no more.  0 (no cap) unless 3 <= w <= 7.
+inline unsigned occupancy_lds(int64_t w) {
+    constexpr int64_t kLdsPerCu = 160 * 1024;
+    if (w < 3 || w > 7) return 0;
+    return static_cast<unsigned>(kLdsPerCu / (w + 1) + 1024);   // in (kLdsPerCu/(w+1), kLdsPerCu/w]
+}
+
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b % 8).  Remap so
 // that XCD k processes the k-th contiguous slice of the logical block range: consecutive block
 // rows then share one L2, which is where their common neighbour columns are cached.

@@ -65,6 +65,7 @@ struct StepArgs {
     double bdt;           // beta / dt
     double ddt;           // damp / dt
     const int32_t *order; // visiting order of the rows (wave-uniform kernels; null = natural)
+    unsigned occ_lds;     // launch only: unused LDS per workgroup that caps the occupancy
 };
 
 // The same arguments with global-address-space pointers (see CVFES_G); the generic row functions

@@ -127,6 +127,7 @@ class MembraneSystem:
         self.row_order = None
         self.color_ptr = self.color_elems = None      # element colouring (EBE variant), on demand
         self.n_colors = self.max_color_elems = 0
+        self.waves_per_simd = 0                       # occupancy cap of the step kernels (0 = none)
         fixed = np.zeros(self.n_nodes, np.uint8)
         fixed[np.asarray(boundary, np.int64)] = 1
         self.fixed = _dev(fixed, torch.uint8, dev)
@@ -245,7 +246,8 @@ class MembraneSystem:
             max_node_elems=self.max_node_elems, uvalues=ptr(self.uvalues), vblk=ptr(self.vblk),
             nnzu=self.nnzu, row_order=ptr(self.row_order), color_ptr=ptr(self.color_ptr),
             color_elems=ptr(self.color_elems), elems=ptr(self.elems), cpos=ptr(self.cpos),
-            n_colors=self.n_colors, max_color_elems=self.max_color_elems)
+            n_colors=self.n_colors, max_color_elems=self.max_color_elems,
+            waves_per_simd=self.waves_per_simd)
 
     def set_variant(self, variant):
         """Select the step kernel variant (names in ``_lib.VARIANTS``); the symmetric ones form the
@@ -301,6 +303,61 @@ class MembraneSystem:
                 raise ValueError("row order must be a permutation of the nodes")
             self.row_order = _dev(perm.astype(np.int32), torch.int32, self.device)
         self._sys = self._make_struct()
+
+    def set_occupancy(self, waves_per_simd=0):
+        """Cap the step kernels that stage nothing in LDS at ``waves_per_simd`` (3..7; 0 = no cap)
+        wavefronts per SIMD.  The results do not depend on it."""
+        w = int(waves_per_simd)
+        if w != 0 and not 3 <= w <= 7:
+            raise ValueError("waves_per_simd must be 0 or 3..7")
+        self.waves_per_simd = w
+        self._sys = self._make_struct()
+
+    def _time_steps(self, load, dt, beta, damp, method, steps):
+        z = np.zeros(3 * self.n_nodes)
+        self.initial_state(z, z, load, dt, beta=beta, damp=damp, method=method)
+        self.step(load, dt, beta=beta, damp=damp, method=method, offset=0, nsteps=2)
+        st = torch.cuda.current_stream(self.device)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        self.step(load, dt, beta=beta, damp=damp, method=method, offset=2, nsteps=steps)
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def autotune_row_order(self, load, dt, beta=0.0, damp=0.0, method="stored", steps=20,
+                           candidates=(("natural", 0), ("strips", 32), ("strips", 64))):
+        """Time ``steps`` steps under each visiting order in ``candidates`` ((order, strip width))
+        and keep the fastest; returns ``{"order", "width", "ms"}``.  The displacements do not depend
+        on the order (every order gives the same result bit for bit), only the time does.  Leaves
+        a fresh zero initial state behind."""
+        best = None
+        for order, width in candidates:
+            self.set_row_order(order, width=width)
+            ms = self._time_steps(load, dt, beta, damp, method, steps)
+            if best is None or ms < best["ms"]:
+                best = {"order": order, "width": width, "ms": ms}
+        self.set_row_order(best["order"], width=best["width"])
+        z = np.zeros(3 * self.n_nodes)
+        self.initial_state(z, z, load, dt, beta=beta, damp=damp, method=method)
+        return best
+
+    def autotune_occupancy(self, load, dt, beta=0.0, damp=0.0, method="stored", steps=10,
+                           candidates=(0, 6, 5, 4)):
+        """Time ``steps`` steps under each occupancy cap in ``candidates`` (waves per SIMD, 0 = no
+        cap) and keep the fastest; returns ``{"waves_per_simd", "ms"}``.  Large rows (many
+        samples, long rings) run faster with fewer rows in flight per XCD; the displacements do not
+        depend on it.  Leaves a fresh zero initial state behind."""
+        best = None
+        for w in candidates:
+            self.set_occupancy(w)
+            ms = self._time_steps(load, dt, beta, damp, method, steps)
+            if best is None or ms < best["ms"]:
+                best = {"waves_per_simd": w, "ms": ms}
+        self.set_occupancy(best["waves_per_simd"])
+        z = np.zeros(3 * self.n_nodes)
+        self.initial_state(z, z, load, dt, beta=beta, damp=damp, method=method)
+        return best
 
     @property
     def stream(self):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 8
+#define CVFES_ABI_VERSION 9
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -87,6 +87,12 @@ typedef struct cvfes_system {
     const int32_t *cpos;        /* [9E] contribution index of element e's block (a, b)          */
     int64_t n_colors;
     int64_t max_color_elems;    /* largest colour (launch bound)                                  */
+    /* Cap on the wavefronts per SIMD of the step, adjoint and product kernels that stage nothing
+     * in LDS (generic, row-load, symmetric), enforced with unused LDS per workgroup: 0 = as many
+     * as the registers allow, else 3..7.  Fewer rows in flight keep a larger share of their
+     * gathered neighbour data in the XCD's L2 when the rows are large (many samples, long rings);
+     * the results do not depend on it. */
+    int64_t waves_per_simd;
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */
@@ -366,7 +372,9 @@ int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void 
 
 /* One step of every mesh (one launch).  Step index for the load tables: counter[0] + k_local
  * (k_local alone when counter is NULL).  n_waves: persistent grid size in wavefronts (0 = 16 per
- * compute unit). */
+ * compute unit).  kinds: the value cvfes_batch_plan returned, plus the caller's launch choices:
+ * bit 5 caps the kernel at 64 registers (8 waves per SIMD, spilling where the paths need more),
+ * bits 6..8 hold a cap on the waves per SIMD (3..7, through unused LDS per workgroup; 0 = none). */
 int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks, int phase,
                      const int64_t *counter, int64_t k_local, int64_t n_waves, int64_t kinds);
 

@@ -46,17 +46,20 @@ def states(systems):
     return [s.u.cpu().numpy().copy() for s in systems]
 
 
-@pytest.mark.parametrize("vw_hint,cap", [(0, False), (1, False), (2, False), (2, True), (0, True)])
-def test_batched_equals_per_mesh(require_gpu, vw_hint, cap):
+@pytest.mark.parametrize("vw_hint,cap,waves", [(0, False, 0), (1, False, 0), (2, False, 0),
+                                               (2, True, 0), (0, True, 0), (2, False, 4),
+                                               (0, True, 7)])
+def test_batched_equals_per_mesh(require_gpu, vw_hint, cap, waves):
     """Every lane-width policy (two samples per lane where even / one everywhere / two only on
-    wave-uniform rows) gives the per-mesh result bit for bit."""
+    wave-uniform rows), with or without the register and waves-per-SIMD caps, gives the per-mesh
+    result bit for bit."""
     from cvfes_amd.batch import BatchedEnsemble
     nsteps = 40
     systems, loads = make_ensemble(nsteps)
     cols = list(zip(*SPECS))
     ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
                           betas=list(cols[6]), damps=list(cols[7]))
-    ens.vw_hint, ens.cap = vw_hint, cap
+    ens.vw_hint, ens.cap, ens.waves_per_simd = vw_hint, cap, waves
     ens.initial_state()
     assert ens.kinds == {0: 31, 1: 14, 2: 15}[vw_hint]
     ens.run(nsteps, graph=False)

@@ -203,6 +203,27 @@ def test_row_order_does_not_change_result(case128, method, variant):
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
+@pytest.mark.parametrize("method,variant", [("stored", "rowload_nt"), ("stored", "sym_ntl"),
+                                            ("stored", "generic"), ("matfree", "generic")])
+def test_occupancy_cap_does_not_change_result(case128, method, variant):
+    """Capping the waves per SIMD (unused LDS per workgroup) gives bit-identical states."""
+    s = case128.sys
+    outs = []
+    for w in (0, 4, 7):
+        s.set_variant(variant)
+        s.set_occupancy(w)
+        try:
+            out, snaps = _run_both(case128, 40, method, beta=1e-6, damp=30.0, dt=3e-6, record=(40,))
+        finally:
+            s.set_occupancy(0)
+            s.set_variant("auto")
+        outs.append(out[40])
+        assert rel_err(out[40], snaps[40]) < REL_U
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    with pytest.raises(ValueError):
+        s.set_occupancy(2)
+
+
 @pytest.mark.parametrize("method,variant", [("stored", "generic"), ("stored", "rowload_nt"),
                                             ("stored", "sym_ntl"), ("matfree", "generic"),
                                             ("matfree", "lds"), ("matfree", "cached"),
