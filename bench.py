@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--case", default="tube-256x400-s128")
     ap.add_argument("--method", default="stored", choices=["stored", "matfree"])
     ap.add_argument("--variant", default="auto", help="step kernel variant (cvfes_amd._lib.VARIANTS)")
+    ap.add_argument("--waves-per-simd", default="auto", choices=["auto", "0", "3", "4", "5", "6", "7"],
+                    help="occupancy cap of the timed step (auto: the fastest of none, 6, 5, 4)")
     ap.add_argument("--row-order", default="auto", choices=["auto", "natural", "strips"],
                     help="row visiting order of the timed step (auto: the fastest of natural and "
                          "strips 32/64 on 20 steps each; the result does not depend on it)")
@@ -184,7 +186,11 @@ def main():
     sysm.set_variant(args.variant)
     total = args.warmup + args.steps
     load = sysm.loads(case.pressures(0, max(total, args.cpu_steps + 1), dt))
-    occ = sysm.autotune_occupancy(load, dt, beta, method=args.method)
+    if args.waves_per_simd == "auto":
+        occ = sysm.autotune_occupancy(load, dt, beta, method=args.method)
+    else:
+        sysm.set_occupancy(int(args.waves_per_simd))
+        occ = {"waves_per_simd": int(args.waves_per_simd)}
     if args.row_order == "auto":
         order = sysm.autotune_row_order(load, dt, beta, method=args.method)
     else:
