@@ -93,7 +93,10 @@ __device__ __forceinline__ void adj_finish(const AdjArgs &a, int32_t row, int64_
             }
             o.v[v] = fixed ? 0.0 : lam;
         }
-        if (active) store_vec<VW>(a.out + off, o);
+        if (active) {   // non-temporal: not read again in this launch
+#pragma unroll
+            for (int v = 0; v < VW; ++v) __builtin_nontemporal_store(o.v[v], a.out + off + v);
+        }
     }
 }
 
