@@ -31,6 +31,7 @@ struct AdjArgs {
     double dt2, bdt, dtdamp;
     int project;
     unsigned occ_lds;      // launch only: unused LDS per workgroup that caps the occupancy
+    bool nt_out;           // write `out` with non-temporal stores (stream_out)
 };
 
 template <int VW>
@@ -93,9 +94,13 @@ __device__ __forceinline__ void adj_finish(const AdjArgs &a, int32_t row, int64_
             }
             o.v[v] = fixed ? 0.0 : lam;
         }
-        if (active) {   // non-temporal: not read again in this launch
+        if (active) {   // not read again in this launch: large outputs are streamed (stream_out)
+            if (a.nt_out) {
 #pragma unroll
-            for (int v = 0; v < VW; ++v) __builtin_nontemporal_store(o.v[v], a.out + off + v);
+                for (int v = 0; v < VW; ++v) __builtin_nontemporal_store(o.v[v], a.out + off + v);
+            } else {
+                store_vec<VW>(a.out + off, o);
+            }
         }
     }
 }
@@ -327,6 +332,7 @@ AdjArgs adjoint_args(const cvfes_system_t *sys, const double *mu1, const double 
     a.dtdamp = dt * damp;
     a.project = project;
     a.occ_lds = occupancy_lds(sys->waves_per_simd);
+    a.nt_out = stream_out(sys->n_nodes, sys->n_smp);
     return a;
 }
 
@@ -387,6 +393,7 @@ extern "C" int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys,
     f.ddt = damp / dt;
     f.order = nullptr;
     f.occ_lds = occupancy_lds(sys->waves_per_simd);
+    f.nt_out = stream_out(sys->n_nodes, sys->n_smp);
     const AdjArgs a = adjoint_args(sys, mu1, mu2, mu_out, obs_u, obs_ref, obs_w, dt, beta, damp, 1);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (method == CVFES_METHOD_MATFREE)

@@ -835,6 +835,7 @@ extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, d
     a.ddt = damp / dt;
     a.order = sys->row_order;
     a.occ_lds = occupancy_lds(sys->waves_per_simd);
+    a.nt_out = stream_out(sys->n_nodes, sys->n_smp);
     const bool mf = method == CVFES_METHOD_MATFREE;
     double *cur = state[0], *prev = state[1], *nxt = state[2];
     for (int64_t n = 0; n < nsteps; ++n) {

@@ -89,6 +89,8 @@ class GPUSolidSolver(PhysicsSolver):
         self.method = getattr(config, "stiffness_method", "stored") or "stored"
         self.updateInterval = update_interval(config)
         self.massLumping = getattr(config, "mass_lumping", "cpu") or "cpu"
+        self.kernelAutotune = bool(getattr(config, "kernel_autotune", True))
+        self.tuned = None
         self.time0 = float(getattr(config, "time", 0.0) or 0.0)
 
         self.dofs = (3 * np.asarray(self.lclNodeIds)[:, None] + np.arange(3)).ravel()
@@ -134,6 +136,12 @@ class GPUSolidSolver(PhysicsSolver):
             bdy_u=float(mesh.bdyU), store_values=(self.method == "stored" and self.updateInterval is None),
             device=self.device)
         self.t = self.time0
+        if self.updateInterval is None and self.kernelAutotune:
+            # occupancy cap of the step kernel, timed on a constant load (the displacements do not
+            # depend on it); the initial state below replaces the tuning steps' state
+            tune = self.system.loads(np.full(16, 1.0))
+            self.tuned = self.system.autotune_occupancy(tune, self.dt, beta=self.beta, damp=self.damp,
+                                                        method=self.method, steps=6)
         load0 = self._load_tables(np.array([self.time0]))
         self.system.initial_state(self.u, self.du, load0[0][1], self.dt, beta=self.beta,
                                   damp=self.damp, method=self.method)
