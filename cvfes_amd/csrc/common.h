@@ -55,7 +55,9 @@ inline unsigned grid_for(int64_t work) {
 // ordinary stores so that the next step finds them in cache.
 constexpr int64_t kStreamOutBytes = int64_t(256) << 20;
 
-inline bool stream_out(int64_t n_nodes, int64_t n_smp) { return n_nodes * n_smp * 24 >= kStreamOutBytes; }
+__host__ __device__ inline bool stream_out(int64_t n_nodes, int64_t n_smp) {
+    return n_nodes * n_smp * 24 >= kStreamOutBytes;
+}
 
 // Unused LDS per 256-thread workgroup (one wave on each SIMD) that caps a kernel at w waves per
 // SIMD: a CU's 160 KiB then holds w such workgroups and no more.  0 (no cap) unless 3 <= w <= 7.
