@@ -95,12 +95,8 @@ __device__ __forceinline__ void adj_finish(const AdjArgs &a, int32_t row, int64_
             o.v[v] = fixed ? 0.0 : lam;
         }
         if (active) {   // not read again in this launch: large outputs are streamed (stream_out)
-            if (a.nt_out) {
-#pragma unroll
-                for (int v = 0; v < VW; ++v) __builtin_nontemporal_store(o.v[v], a.out + off + v);
-            } else {
-                store_vec<VW>(a.out + off, o);
-            }
+            if (a.nt_out) store_vec_nt<VW>(a.out + off, o);
+            else store_vec<VW>(a.out + off, o);
         }
     }
 }

@@ -58,7 +58,7 @@ __device__ __forceinline__ void batch_task(const cvfes_batch_mesh_t &d, int phas
     a.dt2 = d.dt * d.dt;
     a.bdt = d.beta / d.dt;
     a.ddt = d.damp / d.dt;
-    a.nt_out = stream_out(d.sys.n_nodes, d.sys.n_smp);
+    a.nt_out = false;   // ordinary stores: the ensemble measured faster without streaming them
     step_row<MF, BETA, DAMP, VW>(a, row, s, active, n);
 }
 

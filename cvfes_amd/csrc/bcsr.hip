@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
             }
         }
     }
-    finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
+    finish_row<DAMP, VW, NT>(a, row, s, active, step, uo, po, acc);   // NT variants stream u_{n+1}
 }
 
 // Stored step on the symmetric storage: the row's blocks are read from the upper entries
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
                 for (int j = 0; j < 3; ++j) acc[i] += sel<VW>(valid, kv[3 * i + j]) * w[j];
         }
     }
-    finish_row<DAMP, VW>(a, row, s, active, step, uo, po, acc);
+    finish_row<DAMP, VW, NT != 0>(a, row, s, active, step, uo, po, acc);
 }
 
 // Matrix-free step with the row's coefficients loaded once per row (step_row_cached); one

@@ -142,4 +142,26 @@ __device__ __forceinline__ void store_vec(CVFES_G double *__restrict__ p, const 
     }
 }
 
+// Non-temporal stores of VW consecutive samples, as one 16-byte store for VW = 2 (two 8-byte
+// non-temporal stores would write half lines past L2).
+template <int VW>
+__device__ __forceinline__ void store_vec_nt(double *__restrict__ p, const Vec<VW> &r) {
+    if constexpr (VW == 2) {
+        __builtin_nontemporal_store(gdbl2_t{r.v[0], r.v[1]}, reinterpret_cast<gdbl2_t *>(p));
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) __builtin_nontemporal_store(r.v[k], p + k);
+    }
+}
+
+template <int VW>
+__device__ __forceinline__ void store_vec_nt(CVFES_G double *__restrict__ p, const Vec<VW> &r) {
+    if constexpr (VW == 2) {
+        __builtin_nontemporal_store(gdbl2_t{r.v[0], r.v[1]}, reinterpret_cast<CVFES_G gdbl2_t *>(p));
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) __builtin_nontemporal_store(r.v[k], p + k);
+    }
+}
+
 }  // namespace cvfes

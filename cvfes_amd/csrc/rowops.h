@@ -262,12 +262,8 @@ __device__ __forceinline__ void step_update(const SA &a, int32_t row, int64_t s,
         }
         // u_{n+1} is not read again in this launch: large outputs are streamed past L2 (stream_out)
         if (active) {
-            if (a.nt_out) {
-#pragma unroll
-                for (int v = 0; v < VW; ++v) __builtin_nontemporal_store(o.v[v], a.out + off + v);
-            } else {
-                store_vec<VW>(a.out + off, o);
-            }
+            if (a.nt_out) store_vec_nt<VW>(a.out + off, o);
+            else store_vec<VW>(a.out + off, o);
         }
     }
 }
@@ -311,8 +307,10 @@ __device__ __forceinline__ typename VecT<VW>::type sel(bool keep, typename VecT<
 }
 
 // Central-difference update of one block row from its product acc = K (u + bdt (u - up)), with
-// the row's own u/up already loaded; stores u_{n+1} when `active`.
-template <bool DAMP, int VW>
+// the row's own u/up already loaded; stores u_{n+1} when `active`, non-temporally when NTO (a
+// compile-time choice: the row-load and symmetric kernels with non-temporal value loads, which
+// are the forms for large problems, stream their output too).
+template <bool DAMP, int VW, bool NTO = false>
 __device__ __forceinline__ void finish_row(const StepArgs &a, int32_t row, int64_t s, bool active,
                                            int64_t step, const typename VecT<VW>::type (&uo)[3],
                                            const typename VecT<VW>::type (&po)[3],
@@ -342,9 +340,9 @@ __device__ __forceinline__ void finish_row(const StepArgs &a, int32_t row, int64
             un = m != 0.0 ? rhs / m : 0.0;
         }
         if (fixed) un = splat<VW>(a.bdy_u);
-        if (active) {   // streamed when large, as in step_update
+        if (active) {
             V *dst = reinterpret_cast<V *>(a.out + (static_cast<int64_t>(row) * 3 + i) * S + s);
-            if (a.nt_out) __builtin_nontemporal_store(un, dst);
+            if constexpr (NTO) __builtin_nontemporal_store(un, dst);
             else *dst = un;
         }
     }
