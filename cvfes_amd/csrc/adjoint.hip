@@ -127,12 +127,11 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
 // dt^2/m (b1 mu1 - b0 mu2).  The products and updates are the operations of k_step and
 // k_adjoint_step in the same order, so both outputs equal those of the separate launches bit for
 // bit.  The forward and the adjoint may be at different time steps (the reverse sweep of one
-// checkpoint segment runs with the recomputation of the segment before it).  Register target:
-// 5 waves per SIMD (at most 96 VGPRs, no spills in any instance; left alone the stored two-sample
-// instance takes 100 and runs at 4).
+// checkpoint segment runs with the recomputation of the segment before it).  (A register target
+// of 5 waves per SIMD, 94 VGPRs instead of 100 for the stored two-sample instance, measured 2-3 %
+// slower: the loads in flight per wave count for more here.)
 template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
-void k_fused_step(StepArgs f, AdjArgs a, int64_t step) {
+__global__ __launch_bounds__(kBlock) void k_fused_step(StepArgs f, AdjArgs a, int64_t step) {
     int32_t row;
     int64_t s;
     bool active;
