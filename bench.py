@@ -125,6 +125,13 @@ def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads):
            "sample": f"{case.name}: same mesh, {S} samples and inputs; {len(timed)} timed steps after "
                      f"1 warm-up ({float(np.sum(times)):.1f} s of CPU work); K assembled in the "
                      f"reference layout [nnzb][S][3][3] in {t_asm:.1f} s (not timed)"}
+    if threads != 1:
+        # the same step on one thread (the next step of the recurrence, result discarded)
+        t0 = time.perf_counter()
+        co.step(sysm.indptr_h, sysm.indices_h, K, LM, LHS, f1, p[steps], u, up, dt, beta=beta,
+                bdy_dofs=bdy_dofs, threads=1)
+        t1 = time.perf_counter() - t0
+        res["one_thread"] = {"value": S / t1, "unit": "sample-steps/s", "cores": 1, "steps": 1}
     return res, u
 
 
@@ -270,7 +277,9 @@ def main():
         ug = sysm.u.cpu().numpy().reshape(-1, S)
         out["parity_vs_cpu"] = {"steps": args.cpu_steps,
                                 "rel_max_err": float(np.max(np.abs(ug - u_cpu)) / np.max(np.abs(u_cpu)))}
-        print(f"[bench] cpu baseline {cres['value']:.1f} sample-steps/s on {threads} threads; "
+        one = cres.get("one_thread", {}).get("value")
+        print(f"[bench] cpu baseline {cres['value']:.1f} sample-steps/s on {threads} threads"
+              + (f" ({one:.1f} on one)" if one else "") + "; "
               f"parity {out['parity_vs_cpu']['rel_max_err']:.2e}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
