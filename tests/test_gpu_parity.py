@@ -182,6 +182,29 @@ def test_padded_wavefront_rows_match_oracle(require_gpu, method, n_smp):
     assert rel_err(y.reshape(-1, c.S), yref) < 1e-13
 
 
+@pytest.mark.parametrize("n_smp", [1, 2, 3])
+@pytest.mark.parametrize("method", ["stored", "matfree", "matfree-ebe"])
+def test_smallest_sample_counts_match_oracle(require_gpu, method, n_smp):
+    """One, two and three samples: one-lane rows, two samples on one lane, an odd count."""
+    import torch
+    c = Case("tube-64x32-s16", n_smp=n_smp, seed=11)
+    if method == "matfree-ebe":
+        method = "matfree"
+        c.sys.set_variant("ebe")
+    out, snaps = _run_both(c, 120, method, beta=1e-6, damp=20.0, dt=3e-6, record=(1, 40))
+    for k, u in out.items():
+        assert rel_err(u, snaps[k]) < REL_U, f"step {k}"
+    # zero steps leave the state untouched
+    before = [b.clone() for b in c.sys.state]
+    c.sys.step(c.case.pressures(0, 1, 3e-6), 3e-6, beta=1e-6, damp=20.0, method=method, nsteps=0)
+    for b0, b1 in zip(before, c.sys.state):
+        assert torch.equal(b0, b1)
+    x = np.random.default_rng(2).standard_normal((c.N, 3, c.S))
+    y = c.sys.spmv(torch.as_tensor(x).cuda(), method).cpu().numpy()
+    yref = so.multiply_by_vector(c.indptr, c.indices, c.K, x.reshape(-1, c.S))
+    assert rel_err(y.reshape(-1, c.S), yref) < 1e-13
+
+
 @pytest.mark.parametrize("method,variant", [("stored", "rowload_nt"), ("stored", "sym"),
                                             ("stored", "sym_nt"), ("stored", "sym_ntl"),
                                             ("stored", "generic"), ("matfree", "generic")])
