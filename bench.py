@@ -451,6 +451,7 @@ def run_adjoint(args):
     t2 = time.perf_counter()
     fwd, adj = t1 - t0, t2 - t1
     fused_steps = prob.stats.get("fused_steps", 0)
+    fusion_timing = prob.stats.get("fusion_timing_ms")
     # the same reverse sweep with recomputation and adjoint steps in separate kernels
     prob.fuse = False
     prob.forward()
@@ -469,7 +470,7 @@ def run_adjoint(args):
         "forward_s": fwd, "adjoint_s": adj,
         "forward_sample_steps_per_s": S * nsteps / fwd,
         "gradient_max_abs": float(g.abs().max()),
-        "fused_steps": fused_steps,
+        "fused_steps": fused_steps, "fusion_timing_ms": fusion_timing,
         "adjoint_separate_kernels_s": adj_sep, "ratio_separate_kernels": adj_sep / fwd,
         "fused_equals_separate": bool(torch.equal(g, g_sep)),
     }
@@ -543,7 +544,7 @@ def run_combined(args):
     g = prob.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    log(f"adjoint {t2 - t1:.1f} s")
+    log(f"adjoint {t2 - t1:.1f} s ({prob.stats.get('fused_steps', 0)} fused steps)")
     fwd, adj = t1 - t0, t2 - t1
     peak = torch.cuda.max_memory_allocated() / 1e9
     out = {
@@ -557,6 +558,8 @@ def run_combined(args):
                    "checkpoints": [C1, C2], "adjoint_operator": "matfree"},
         "step_operators": per_step, "stored_vs_matfree_rel_max_err": agree,
         "forward_s": fwd, "adjoint_s": adj, "adjoint_over_forward": adj / fwd,
+        "fused_steps": prob.stats.get("fused_steps", 0),
+        "fusion_timing_ms": prob.stats.get("fusion_timing_ms"),
         "peak_device_GB": peak, "loss_mean": float(loss.mean()),
         "gradient_max_abs": float(g.abs().max()),
     }

@@ -11,9 +11,11 @@ from the checkpoint, then ``cvfes_adjoint_step`` (the transposed step) and ``cvf
 pair of states per checkpoint plus one segment of states.  With ``inner_every = C2`` the sweep is
 two-level: a segment of ``C`` steps is first recomputed keeping a pair every ``C2`` steps, and
 only ``C2`` states are held at a time, so memory falls from ``2 n/C + C`` to
-``2 n/C + 2 C/C2 + C2`` states for one more forward pass.  Observations may live in host memory
-(copied to the device when used).  Recomputation replays the same arithmetic, so the gradient does
-not depend on the checkpoint scheme.  ``EnsembleLoss`` wraps it as a ``torch.autograd.Function`` of
+``2 n/C + 2 C/C2 + C2`` states for one more forward pass.  The innermost segments can be
+pipelined: the segment before the current one is recomputed during the current one's adjoint
+steps, a recomputed step and an adjoint step per fused kernel, in the buffers the current one
+releases.  Observations may live in host memory (copied to the device when used).  Recomputation
+replays the same arithmetic, so the gradient does not depend on the checkpoint scheme.  ``EnsembleLoss`` wraps it as a ``torch.autograd.Function`` of
 ``theta = log E [E, S]``.
 
 All device work is in the native kernels; torch provides the buffers, the per-sample loss
@@ -45,14 +47,18 @@ class AdjointProblem:
         in (preferably pinned) host memory.
     checkpoint_every : checkpoint interval ``C`` (1 = keep every state, no recomputation).
     inner_every : second-level interval ``C2`` inside a segment (``None`` = one level).
-    fuse : with one level, recompute the segment before the current one during the current one's
-        reverse sweep, each recomputed step in the same kernel as an adjoint step (the operator is
-        read once for both; needs the generic forward kernel and ``C - 1`` more states of memory).
+    fuse : recompute the segment before the current one (of the innermost level) during the
+        current one's reverse sweep, each recomputed step in the same kernel as an adjoint step
+        (the operator is read once for both; needs the generic forward kernel and one more state
+        of memory).  ``"auto"`` times three fused kernels against three forward + adjoint pairs
+        on the problem's own states first and keeps the faster form (the fused kernel carries
+        two columns of states and loses where they overflow L2); the gradient is the same bit
+        for bit either way.
     """
 
     def __init__(self, system: MembraneSystem, load: LoadTables, nsteps, dt, beta=0.0, damp=0.0,
                  method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None,
-                 inner_every=None, fuse=True):
+                 inner_every=None, fuse="auto"):
         self.sys = system
         self.load = load
         self.nsteps = int(nsteps)
@@ -65,7 +71,7 @@ class AdjointProblem:
         self.u_obs = u_obs
         self.C = max(1, int(checkpoint_every))
         self.C2 = None if inner_every is None or int(inner_every) >= self.C else max(1, int(inner_every))
-        self.fuse = bool(fuse)
+        self.fuse = fuse if fuse == "auto" else bool(fuse)
         self._obs_dev = None
         N = system.n_nodes
         self.u0 = np.zeros(3 * N) if u0 is None else np.asarray(u0, np.float64)
@@ -146,9 +152,14 @@ class AdjointProblem:
                                                 ptr(obs_ref), ptr(w), self.dt, self.beta, self.damp))
 
     def fused(self):
-        """Whether the reverse sweep pairs recomputation and adjoint steps in one kernel."""
-        if not self.fuse or self.C2 is not None or self.C < 2 or len(self.checkpoints) < 2:
+        """Whether the reverse sweep can pair recomputation and adjoint steps in one kernel (with
+        ``fuse = "auto"`` it does so where that is faster, ``stats["fused"]``)."""
+        leaf = self.C if self.C2 is None else self.C2
+        if not self.fuse or leaf < 2 or (self.C2 is None and len(self.checkpoints) < 2):
             return False
+        return self._generic_forward_kernel()
+
+    def _generic_forward_kernel(self):
         sysm = self.sys
         kern = sysm.lib.cvfes_step_kernel(ctypes.byref(sysm._sys), sysm.method_id(self.method))
         return kern == VARIANT_GENERIC
@@ -177,24 +188,35 @@ class AdjointProblem:
         self._mu = [mu[0], zeros.clone(), mu[1]]              # mu_{n+1}, mu_{n+2}, spare
         self._w, self._grad = w, grad
         leaf = self.C if self.C2 is None else self.C2
-        self._seg_bufs = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(leaf - 1)]
+        fused = self.fused()
+        # states of the innermost segment (the pipelined sweep also holds the next one's, in the
+        # buffers the current one releases, plus one)
+        self._pool = [torch.empty(shape, dtype=torch.float64, device=dev)
+                      for _ in range(leaf if fused else leaf - 1)]
         self._tmp = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)] \
             if self.C2 is not None else []
         self.stats["fused_steps"] = 0
-        if self.fused():
-            self._reverse_pipelined(shape)
+        if fused and self.fuse == "auto":
+            last = max(self.checkpoints)
+            fused = self._fusion_pays(self.checkpoints[last], last)
+        self.stats["fused"] = fused
+        if fused and self.C2 is None:
+            self._reverse_pipelined(self.checkpoints, self.C, N)
         else:
             for s0 in sorted(self.checkpoints, reverse=True):
                 s1 = min(s0 + self.C, N)
                 if self.C2 is None:
                     self._reverse_leaf(s0, s1, self.checkpoints[s0])
+                    continue
+                pairs = self._recompute_pairs(s0, s1, self.checkpoints[s0], self.C2)
+                if fused:
+                    self._reverse_pipelined(pairs, self.C2, s1)
                 else:
-                    pairs = self._recompute_pairs(s0, s1, self.checkpoints[s0], self.C2)
                     for t0 in sorted(pairs, reverse=True):
                         self._reverse_leaf(t0, min(t0 + self.C2, s1), pairs[t0])
-                    del pairs
+                del pairs
         mu_n1 = self._mu[0]
-        self._seg_bufs = self._tmp = None
+        self._pool = self._tmp = None
         sysm.state = None
         # initial state: u_{-1} depends on K through (u0 + beta du0)
         x0 = self.u0 + self.beta * self.du0
@@ -204,6 +226,39 @@ class AdjointProblem:
             x0_d = torch.as_tensor(np.repeat(x0.reshape(-1, 3)[:, :, None], S, axis=2)).to(dev)
             self._grad_call(lam_m1, x0_d, None, 0.5 * self.dt * self.dt, grad)
         return grad
+
+    def _fusion_pays(self, pair, n):
+        """Time three fused kernels against three forward + adjoint pairs (after one of each) on
+        the checkpoint ``pair`` of step ``n`` and the current adjoint state, writing only into
+        scratch buffers of the pool; True if the fused kernel is faster."""
+        sysm = self.sys
+        u, up = pair
+        a, b = self._pool[0], self._pool[1]
+        mu_n1, mu_n2, _ = self._mu
+
+        def separate():
+            sysm.state = [u, up, a]
+            self._advance(n, 1)
+            self._adjoint_call(mu_n1, mu_n2, b)
+
+        def fused():
+            self._fused_call(u, up, a, n, mu_n1, mu_n2, b)
+
+        st = torch.cuda.current_stream(sysm.device)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        separate()
+        fused()
+        ev[0].record(st)
+        for _ in range(3):
+            separate()
+        ev[1].record(st)
+        for _ in range(3):
+            fused()
+        ev[2].record(st)
+        ev[2].synchronize()
+        t_sep, t_fused = ev[0].elapsed_time(ev[1]) / 3, ev[1].elapsed_time(ev[2]) / 3
+        self.stats["fusion_timing_ms"] = {"separate": t_sep, "fused": t_fused}
+        return t_fused < t_sep
 
     def _recompute_pairs(self, s0, s1, pair, C2):
         """Forward from the pair (u_{s0}, u_{s0-1}) to step s1 - 1, keeping (u_k, u_{k-1}) for
@@ -220,40 +275,41 @@ class AdjointProblem:
                 pairs[n + 1] = (cur.clone(), prev.clone())
         return pairs
 
-    def _recompute(self, s0, s1, pair, bufs, upto=None):
+    def _recompute(self, s0, s1, pair, free):
         """States u_{s0-1} .. u_{s1-1} from the pair (u_{s0}, u_{s0-1}) by plain steps into
-        ``bufs``; returns {n: state}.  ``upto`` stops after producing u_{upto} (default s1 - 1)."""
+        buffers taken from the list ``free``; returns {n: state}."""
         sysm = self.sys
         u_s0, u_sm1 = pair
         states = {s0 - 1: u_sm1, s0: u_s0}
-        last = s1 - 1 if upto is None else upto
-        for n in range(s0, last):
-            sysm.state = [states[n], states[n - 1], bufs[n - s0]]
+        for n in range(s0, s1 - 1):
+            out = free.pop()
+            sysm.state = [states[n], states[n - 1], out]
             self._advance(n, 1)
-            states[n + 1] = bufs[n - s0]
+            states[n + 1] = out
         return states
 
-    def _reverse_pipelined(self, shape):
-        """One-level reverse sweep in which the segment before the current one is recomputed
-        during the current one's adjoint steps, a recomputed step and an adjoint step per fused
-        kernel; the arithmetic of both is that of the separate kernels."""
+    def _reverse_pipelined(self, pairs, C, end):
+        """Reverse sweep over the segments that start at the keys of ``pairs`` (one every ``C``
+        steps, the last ending at ``end``) in which the segment before the current one is
+        recomputed during the current one's adjoint steps, one recomputed step and one adjoint
+        step per fused kernel; the arithmetic of both is that of the separate kernels.  The
+        state u_n of the current segment is not read again once step n's gradient and adjoint
+        kernels are issued (all on one stream), so its buffer takes a recomputed state next:
+        ``C`` buffers hold both segments."""
         sysm = self.sys
-        dev = sysm.device
-        N = self.nsteps
-        starts = sorted(self.checkpoints, reverse=True)
-        bufs = [self._seg_bufs,
-                [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]]
-        cur = self._recompute(starts[0], min(starts[0] + self.C, N), self.checkpoints[starts[0]], bufs[0])
+        free = list(self._pool)
+        starts = sorted(pairs, reverse=True)
+        cur = self._recompute(starts[0], min(starts[0] + C, end), pairs[starts[0]], free)
         mu_n1, mu_n2, spare = self._mu
         fused = 0
         for idx, s0 in enumerate(starts):
-            s1 = min(s0 + self.C, N)
+            s1 = min(s0 + C, end)
             nxt = starts[idx + 1] if idx + 1 < len(starts) else None
+            fn = fend = 0
             if nxt is not None:
-                nbuf = bufs[(idx + 1) % 2]
-                u_c, up_c = self.checkpoints[nxt]
+                u_c, up_c = pairs[nxt]
                 nstates = {nxt - 1: up_c, nxt: u_c}
-                fn, fend = nxt, min(nxt + self.C, N) - 1        # forward steps fn .. fend - 1
+                fn, fend = nxt, min(nxt + C, end) - 1            # forward steps fn .. fend - 1
             for n in range(s1 - 1, s0 - 1, -1):
                 self._grad_call(mu_n1, cur[n], cur[n - 1] if self.beta != 0.0 else None,
                                 self.dt * self.dt, self._grad)
@@ -262,8 +318,8 @@ class AdjointProblem:
                 obs_u = cur[n] if n in self.obs_index else None
                 ref = self._obs(n) if obs_u is not None else None
                 w = self._w if obs_u is not None else None
-                if nxt is not None and fn < fend:
-                    out = nbuf[fn - nxt]
+                if fn < fend:
+                    out = free.pop()
                     self._fused_call(nstates[fn], nstates[fn - 1], out, fn, mu_n1, mu_n2, spare,
                                      obs_u, ref, w)
                     nstates[fn + 1] = out
@@ -272,15 +328,18 @@ class AdjointProblem:
                 else:
                     self._adjoint_call(mu_n1, mu_n2, spare, obs_u, ref, w)
                 mu_n2, mu_n1, spare = mu_n1, spare, mu_n2
+                if n > s0:
+                    free.append(cur[n])                          # a recomputed state, done with
+            while fn < fend:                                     # the rest of the recomputation
+                out = free.pop()
+                sysm.state = [nstates[fn], nstates[fn - 1], out]
+                self._advance(fn, 1)
+                nstates[fn + 1] = out
+                fn += 1
             if nxt is not None:
-                while fn < fend:                                 # the rest of the recomputation
-                    sysm.state = [nstates[fn], nstates[fn - 1], nbuf[fn - nxt]]
-                    self._advance(fn, 1)
-                    nstates[fn + 1] = nbuf[fn - nxt]
-                    fn += 1
                 cur = nstates
         self._mu = [mu_n1, mu_n2, spare]
-        self.stats["fused_steps"] = fused
+        self.stats["fused_steps"] += fused
 
     def _reverse_leaf(self, s0, s1, pair):
         """Recompute u_{s0+1} .. u_{s1-1} from (u_{s0}, u_{s0-1}), then run the adjoint and gradient
@@ -289,9 +348,9 @@ class AdjointProblem:
         u_s0, u_sm1 = pair
         states = {s0 - 1: u_sm1, s0: u_s0}
         for i, n in enumerate(range(s0, s1 - 1)):
-            sysm.state = [states[n], states[n - 1], self._seg_bufs[i]]
+            sysm.state = [states[n], states[n - 1], self._pool[i]]
             self._advance(n, 1)
-            states[n + 1] = self._seg_bufs[i]
+            states[n + 1] = self._pool[i]
         mu_n1, mu_n2, spare = self._mu
         for n in range(s1 - 1, s0 - 1, -1):
             # gradient of step n: mu_{n+1}, x_n = u_n + (beta/dt)(u_n - u_{n-1})

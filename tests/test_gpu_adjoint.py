@@ -88,9 +88,9 @@ def test_adjoint_matches_torch_autograd(require_gpu, method, beta, damp, C, u0, 
 def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
     """The pipelined sweep (recomputation of a segment fused with the adjoint steps of the next)
     gives the gradient of the separate kernels bit for bit, with beta, damping, an initial
-    displacement, observations on both kinds of step and a checkpoint interval that does not
-    divide the step count, also with the kernels capped at 4 waves per SIMD; and it does fuse
-    steps."""
+    displacement, observations on both kinds of step and checkpoint intervals that do not
+    divide the step count, with one and with two checkpoint levels, also with the kernels capped
+    at 4 waves per SIMD; and it does fuse steps."""
     from cvfes_amd.adjoint import AdjointProblem
     nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 7)
     nsteps, dt = 75, 4e-6
@@ -100,18 +100,21 @@ def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
     u_obs = torch.as_tensor(1e-5 * rng.standard_normal((len(obs), nodes.shape[0], 3, S))).cuda()
     u0 = 1e-6 * rng.standard_normal(3 * nodes.shape[0])
     grads = []
-    for fuse, waves in ((False, 0), (True, 0), (True, 4)):
-        sysm.set_occupancy(waves)
-        prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0, method=method,
-                              obs_steps=obs, u_obs=u_obs, checkpoint_every=11, u0=u0, fuse=fuse)
-        prob.set_theta(torch.as_tensor(theta).cuda())
-        prob.forward()
-        grads.append(prob.backward().cpu().numpy())
-        assert prob.fused() == fuse
-        assert (prob.stats["fused_steps"] > 50) == fuse
+    for C, C2, n_fused in ((11, None, 59), (30, 7, 47)):      # fused steps of 75 steps
+        for fuse, waves in ((False, 0), (True, 0), (True, 4)):
+            sysm.set_occupancy(waves)
+            prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0,
+                                  method=method, obs_steps=obs, u_obs=u_obs, checkpoint_every=C,
+                                  inner_every=C2, u0=u0, fuse=fuse)
+            prob.set_theta(torch.as_tensor(theta).cuda())
+            prob.forward()
+            grads.append(prob.backward().cpu().numpy())
+            assert prob.fused() == fuse
+            assert prob.stats["fused_steps"] == (n_fused if fuse else 0)
     sysm.set_occupancy(0)
     assert np.max(np.abs(grads[0])) > 0
-    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+    for g in grads[1:]:
+        assert np.array_equal(grads[0], g)
 
 
 def test_gradient_independent_of_checkpoint_interval(require_gpu):

@@ -328,3 +328,103 @@ def test_update_interval_selects_linear_mode_from_the_run_length():
     cfg = CVConfig(read_cfg(os.path.join(os.path.dirname(__file__), "..", "cvfes_amd", "cfg",
                                          "tube_64x32_s16.cfg")))
     assert update_interval(cfg.solver) is None                 # the shipped cfg: linear mode
+
+
+# ---------------------------------------------------------------- adjoint sweep schedules -------
+class _ToySystem:
+    """CPU stand-in for the device system: the state rotation of ``MembraneSystem.step``."""
+
+    def __init__(self, n, s):
+        import torch
+        self.n_nodes, self.n_smp, self.n_elms = n, s, n
+        self.device = torch.device("cpu")
+        self.state = None
+
+    def state_shape(self):
+        return (self.n_nodes, 3, self.n_smp)
+
+    @property
+    def u(self):
+        return self.state[0]
+
+    @property
+    def up(self):
+        return self.state[1]
+
+
+def _toy_problem(nsteps, C, C2, fuse, obs, u_obs, beta):
+    """An ``AdjointProblem`` whose kernels are a small linear recurrence on CPU tensors: a forward
+    step, its adjoint, the gradient accumulation and the fused pair (forward + adjoint), each the
+    same arithmetic as the separate calls."""
+    import torch
+    from cvfes_amd.adjoint import AdjointProblem
+
+    def fwd(u, up, n):
+        return 1.6 * u - 0.7 * up + 0.1 * torch.roll(u, 1, 0) + 1e-2 * math.cos(0.3 * n)
+
+    def adj(mu1, mu2, out, obs_u, ref, w):
+        r = 1.6 * mu1 - 0.7 * mu2 + 0.1 * torch.roll(mu1, -1, 0)
+        if obs_u is not None:
+            r = r + w * (obs_u - (0.0 if ref is None else ref))
+        out.copy_(r)
+
+    class Toy(AdjointProblem):
+        def _generic_forward_kernel(self):
+            return True
+
+        def _start(self):
+            z = torch.linspace(-1e-3, 1e-3, 3 * self.sys.n_nodes * self.sys.n_smp,
+                               dtype=torch.float64).reshape(self.sys.state_shape())
+            self.sys.state = [z.clone(), 0.5 * z, torch.empty_like(z)]
+
+        def _advance(self, offset, n):
+            for k in range(n):
+                st = self.sys.state
+                st[2].copy_(fwd(st[0], st[1], offset + k))
+                self.sys.state = [st[2], st[0], st[1]]
+
+        def _adjoint_call(self, mu1, mu2, out, obs_u=None, obs_ref=None, w=None, project=1):
+            adj(mu1, mu2, out, obs_u, obs_ref, w)
+
+        def _fused_call(self, u, up, out, n, mu1, mu2, mu_out, obs_u=None, obs_ref=None, w=None):
+            assert out is not u and out is not up and out is not obs_u
+            out.copy_(fwd(u, up, n))
+            adj(mu1, mu2, mu_out, obs_u, obs_ref, w)
+
+        def _grad_call(self, mu, xc, xp, nu_scale, grad):
+            g = (mu * xc).sum(dim=1)
+            if xp is not None:
+                g = g + 0.25 * (mu * xp).sum(dim=1)
+            grad += nu_scale * g
+
+    sysm = _ToySystem(7, 4)
+    return Toy(sysm, None, nsteps, 0.5, beta=beta, obs_steps=obs, u_obs=u_obs,
+               checkpoint_every=C, inner_every=C2, fuse=fuse)
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.3])
+def test_reverse_sweep_schedules_agree_on_a_toy_recurrence(beta):
+    """Every checkpoint schedule (one or two levels, the recomputation fused with the adjoint
+    steps or not, intervals that do not divide the run) replays the same arithmetic on the same
+    states, so the gradients agree bit for bit; a buffer handed to the next segment's
+    recomputation while its state is still needed would change them."""
+    import torch
+    nsteps, obs = 75, [8, 30, 31, 64, 75]
+    u_obs = torch.linspace(-1, 1, len(obs) * 7 * 3 * 4, dtype=torch.float64).reshape(len(obs), 7, 3, 4)
+    ref = None
+    # fused steps: every adjoint step of a segment that has an earlier segment to recompute,
+    # up to that segment's length - 1 (e.g. C = 11: 9 in [66, 75) + 5 x 10 = 59; C = 30 with
+    # C2 = 7: 7 in [60, 75) + 20 + 20 = 47)
+    for C, C2, fuse, n_fused in ((1, None, False, 0), (11, None, False, 0), (11, None, True, 59),
+                                 (75, None, True, 0), (1000, None, True, 0), (30, 7, False, 0),
+                                 (30, 7, True, 47), (40, 9, True, 52), (30, 1, True, 0),
+                                 (2, None, True, 37), (3, 2, True, 25)):
+        prob = _toy_problem(nsteps, C, C2, fuse, obs, u_obs, beta)
+        loss = prob.forward()
+        g = prob.backward(torch.ones(4, dtype=torch.float64))
+        assert prob.stats["fused_steps"] == n_fused, (C, C2, fuse, prob.stats["fused_steps"])
+        if ref is None:
+            ref = (loss, g)
+            assert float(g.abs().max()) > 0
+        else:
+            assert torch.equal(loss, ref[0]) and torch.equal(g, ref[1]), (C, C2, fuse)
