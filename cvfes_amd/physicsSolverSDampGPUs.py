@@ -147,6 +147,7 @@ class GPUSolidSolver(PhysicsSolver):
                                   damp=self.damp, method=self.method)
         self.pressure = float(self.waveform(self.time0))
         self._step_load = None
+        self._solve_tables = None
 
     # ------------------------------------------------------------------------------------------
     def _load_tables(self, times):
@@ -189,8 +190,14 @@ class GPUSolidSolver(PhysicsSolver):
 
     def Solve(self, t, dt):
         """One step of all samples with the load set by the last ``RefreshContext``."""
-        runs = self._load_tables(np.array([self.t]))
-        n, tables = runs[0]
+        if self.traction is None:
+            # one resident single-step table; the pressure is set by a fill on the launch stream,
+            # so there is no allocation and no host-to-device copy per step
+            if self._solve_tables is None:
+                self._solve_tables = self.system.loads(np.zeros(1))
+            tables = self._solve_tables
+        else:
+            tables = self._load_tables(np.array([self.t]))[0][1]
         tables.p.fill_(self.pressure)
         self._advance(tables, dt)
         self._step_load = tables
