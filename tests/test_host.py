@@ -428,3 +428,14 @@ def test_reverse_sweep_schedules_agree_on_a_toy_recurrence(beta):
             assert float(g.abs().max()) > 0
         else:
             assert torch.equal(loss, ref[0]) and torch.equal(g, ref[1]), (C, C2, fuse)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The measurement is one GPU's share; a multi-GPU request exits with the reason (no GPU
+    touched)."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
