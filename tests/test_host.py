@@ -439,3 +439,39 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_bench_cpu_baseline_is_the_reference_step():
+    """The CPU baseline leg of bench.py (C restatement in the reference layout, timed on the host)
+    advances the same recurrence as the NumPy oracle, on a small damped tube."""
+    import sys
+    import types
+    import torch
+    sys.path.insert(0, ROOT)
+    import bench
+    from oracle import solid_oracle as so
+
+    nodes, el, bdy = tube_geometry(16, 6, 0.5, 2.0)
+    N, S, nu, dt, beta, steps = nodes.shape[0], 4, 0.45, 2e-6, 1e-6, 5
+    rng = np.random.default_rng(4)
+    elmE = 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
+    h = np.full((N, S), 0.07)
+    indptr, indices = so.sparse_info_loops(el, N)
+    _, gth = so.gauss_properties(np.full((N, S), 7e6), h, el)
+    coefK = so.coef_k_element_E(gth, elmE)
+    _, K, f1, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(nu), coefK, gth, 1.0,
+                                         indptr, indices)
+    pres = 1.33e4 * (1.0 + 0.1 * np.sin(np.arange(steps + 1)))
+    sysm = types.SimpleNamespace(indptr_h=indptr, indices_h=indices, n_nodes=N,
+                                 f1=torch.as_tensor(f1), lumped_mass_dofs=lambda: LM)
+    case = types.SimpleNamespace(n_smp=S, name="tube-16x6", nu=nu,
+                                 pressures=lambda n0, n, dt_: pres[n0:n0 + n])
+    res, u = bench.cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, 2)
+    bdy_dofs = (3 * bdy[:, None] + np.arange(3)).ravel()
+    z = np.zeros(3 * N)
+    uo, _, _ = so.run(indptr, indices, K, LM, f1, pres[:steps], z, z, dt, beta=beta,
+                      bdy_dofs=bdy_dofs)
+    assert np.max(np.abs(uo)) > 0
+    assert np.max(np.abs(u - uo)) / np.max(np.abs(uo)) < 1e-12
+    assert res["kind"] == "port" and res["cores"] == 2 and res["value"] > 0
+    assert res["one_thread"]["cores"] == 1 and res["one_thread"]["value"] > 0
