@@ -265,6 +265,7 @@ class SolverConfig:
         self.dt_f = _get(sec, "dt_f", _f, None)
         self.update_interval = _get(sec, "stiffness_update_interval", _i, 1 << 62)
         self.kernel_autotune = _get(sec, "kernel_autotune", _b, True)
+        self.finite_check_every = _get(sec, "finite_check_every", _i, 100)
         self.constant_pressure = _get(sec, "constant_pressure", _f, 0.0)
         self.constant_T = _get(sec, "constant_T", _f, 0.0)
         self.restart = _get(sec, "restart", _b, False)

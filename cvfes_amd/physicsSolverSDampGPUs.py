@@ -90,6 +90,9 @@ class GPUSolidSolver(PhysicsSolver):
         self.updateInterval = update_interval(config)
         self.massLumping = getattr(config, "mass_lumping", "cpu") or "cpu"
         self.kernelAutotune = bool(getattr(config, "kernel_autotune", True))
+        # Solve() checks the displacement for non-finite values every this many steps (0 = never)
+        self.finiteCheckEvery = int(getattr(config, "finite_check_every", 100) or 0)
+        self._nSolve = 0
         self.tuned = None
         self.time0 = float(getattr(config, "time", 0.0) or 0.0)
 
@@ -201,6 +204,9 @@ class GPUSolidSolver(PhysicsSolver):
         tables.p.fill_(self.pressure)
         self._advance(tables, dt)
         self._step_load = tables
+        self._nSolve += 1
+        if self.finiteCheckEvery > 0 and self._nSolve % self.finiteCheckEvery == 0:
+            self.CheckFinite(dt)
 
     def _advance(self, tables, dt):
         if self.updateInterval is None:
@@ -215,7 +221,18 @@ class GPUSolidSolver(PhysicsSolver):
         for n, tables in self._load_tables(times):
             self._advance(tables, dt)
         self.t = float(t0 + nsteps * dt)
+        self.CheckFinite(dt)
         return self.system.u
+
+    def CheckFinite(self, dt=None):
+        """Raise ``FloatingPointError`` if the displacement holds a NaN or an infinity (one
+        reduction and one synchronisation).  An explicit step past its stability bound grows
+        without limit, so this is where a too-large ``dt`` shows."""
+        if not bool(torch.isfinite(self.system.u).all()):
+            step = "" if dt is None else f" with dt = {dt:g}"
+            raise FloatingPointError(
+                f"the displacement is not finite at t = {self.t:g}{step}: the explicit step is "
+                "unstable (dt above the stability bound of the mesh, modulus and damping?)")
 
     # ------------------------------------------------------------------------------------------
     def GetDisplacement(self):

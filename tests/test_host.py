@@ -475,3 +475,37 @@ def test_bench_cpu_baseline_is_the_reference_step():
     assert np.max(np.abs(u - uo)) / np.max(np.abs(uo)) < 1e-12
     assert res["kind"] == "port" and res["cores"] == 2 and res["value"] > 0
     assert res["one_thread"]["cores"] == 1 and res["one_thread"]["value"] > 0
+
+
+def test_solver_reports_a_non_finite_displacement():
+    """``Solve`` checks the displacement every ``finite_check_every`` steps and raises
+    ``FloatingPointError`` once it holds a NaN (a stand-in system on the CPU, NaN from step 7)."""
+    import types
+    import torch
+    from cvfes_amd.physicsSolverSDampGPUs import GPUSolidSolver
+
+    pressures = []
+
+    class Sys:
+        u = torch.zeros(4, 3, 2, dtype=torch.float64)
+
+        def step(self, tables, dt, **kw):
+            pressures.append(float(tables.p[0]))
+            if len(pressures) == 7:
+                self.u[1, 2, 0] = float("nan")
+
+    s = object.__new__(GPUSolidSolver)
+    s.system, s.traction, s.updateInterval = Sys(), None, None
+    s.beta = s.damp = 0.0
+    s.method = "stored"
+    s._solve_tables = types.SimpleNamespace(p=torch.zeros(1, dtype=torch.float64))
+    s.t, s.pressure, s.finiteCheckEvery, s._nSolve = 0.0, 5.0, 4, 0
+    for _ in range(7):
+        s.Solve(0.0, 1e-6)                       # checked after step 4: finite
+    with pytest.raises(FloatingPointError, match="not finite"):
+        s.Solve(0.0, 1e-6)                       # checked after step 8
+    assert pressures == [5.0] * 8
+    s.system.u.zero_()
+    s.CheckFinite(1e-6)
+    cfg = CVConfig(read_cfg(os.path.join(CFG_DIR, "tube_64x32_s16.cfg")))
+    assert cfg.solver.finite_check_every == 100
