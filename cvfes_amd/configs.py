@@ -19,7 +19,8 @@ geometry; both end rings fixed.
 
 Time steps: explicit central differences are stable for ``dt^2 lam + 2 dt beta lam < 4`` with
 ``lam = lambda_max(M^-1 K)``.  With the CPU path's lumped mass the 64x32 tube has
-``lam = 1.54e11 s^-2`` (dt_crit = 5.09e-6 s), so its step is 4e-6 s instead of 1e-5 s.  The
+``lam = 1.62e11 s^-2`` at its stiffest of 16 samples (dt_crit = 4.98e-6 s; 1.50e11-1.54e11 for
+the first four), so its step is 4e-6 s instead of 1e-5 s.  The
 102,400-node tube has ``lam = 1.36e12 s^-2``; with beta = 1e-4 its bound is 1.47e-8 s and it runs at
 1.2e-8 s.  The 49,152-node tube (no damping) has ``lam = 6.80e11 s^-2``, bound 2.43e-6 s, step
 2e-6 s.  (Largest eigenvalues from ARPACK on the assembled pencil, first samples.)  Cases with
@@ -113,7 +114,7 @@ class EnsembleCase:
 CASES = {
     c.name: c for c in [
         EnsembleCase("tube-64x32-s16", 64, 32, 16, radius=0.3, length=3.0, dt=4.0e-6, nsteps=1000,
-                     seed=1, notes="specified dt 1e-5 exceeds dt_crit 5.09e-6"),
+                     seed=1, notes="specified dt 1e-5 exceeds dt_crit 4.98e-6"),
         EnsembleCase("tube-256x400-s128", 256, 400, 128, radius=0.5, length=6.0, beta=1.0e-4,
                      dt=1.2e-8, nsteps=2000, seed=2,
                      notes="lam 1.358e12; dt_crit(beta 1e-4) = 1.47e-8"),

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CFG = os.path.join(ROOT, "cvfes_amd", "cfg", "tube_64x32_s16.cfg")
 REL = 1e-10
-# 2^-18 s (< the 5.09e-6 s stability bound): a power of two, so `np.arange(endtime, step=dt)` in the
+# 2^-18 s (< the 4.98e-6 s stability bound): a power of two, so `np.arange(endtime, step=dt)` in the
 # time loop (solver.py:75) gives exactly endtime/dt steps of exactly dt.
 DT = 2.0 ** -18
 

@@ -509,3 +509,51 @@ def test_solver_reports_a_non_finite_displacement():
     s.CheckFinite(1e-6)
     cfg = CVConfig(read_cfg(os.path.join(CFG_DIR, "tube_64x32_s16.cfg")))
     assert cfg.solver.finite_check_every == 100
+
+
+def test_critical_dt_power_iteration_and_bound():
+    """``MembraneSystem.critical_dt`` (power iteration on M^-1 K) on a stand-in whose product is
+    the oracle's: its eigenvalue approaches ARPACK's largest from below, and the returned step is
+    the stability limit of the damped central-difference recurrence, whose characteristic roots
+    leave the unit circle just above it."""
+    import types
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    import torch
+    from cvfes_amd.system import MembraneSystem
+    from oracle import solid_oracle as so
+
+    nodes, el, bdy = tube_geometry(32, 16, 0.3, 1.5)
+    N, S = nodes.shape[0], 4
+    rng = np.random.default_rng(7)
+    E = 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
+    h = np.full((N, S), 0.07)
+    indptr, indices = so.sparse_info_loops(el, N)
+    _, gth = so.gauss_properties(np.full((N, S), 7e6), h, el)
+    coefK = so.coef_k_element_E(gth, E)
+    _, K, f1, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(0.5), coefK, gth, 1.0,
+                                         indptr, indices)
+    # largest eigenvalue of the pencil (K, M) over the samples, by ARPACK on M^-1/2 K M^-1/2
+    row = np.repeat(np.arange(N), indptr[1:] - indptr[:-1])
+    r = np.broadcast_to(3 * row[:, None, None] + np.arange(3)[None, :, None], (row.size, 3, 3))
+    c = np.broadcast_to(3 * indices[:, None, None] + np.arange(3)[None, None, :], (row.size, 3, 3))
+    lam_exact = 0.0
+    for s in range(S):
+        A = sp.csr_matrix((K[:, s].ravel(), (r.ravel(), c.ravel())), shape=(3 * N, 3 * N))
+        d = sp.diags(1.0 / np.sqrt(LM[:, s]))
+        lam_exact = max(lam_exact, sla.eigsh(d @ A @ d, k=1, which="LA",
+                                             return_eigenvectors=False)[0])
+    stand_in = types.SimpleNamespace(
+        device=torch.device("cpu"), lm=torch.as_tensor(LM[0::3].copy()),
+        state_shape=lambda: (N, 3, S),
+        spmv=lambda x, method: torch.as_tensor(so.multiply_by_vector(
+            indptr, indices, K, x.reshape(3 * N, S).numpy()).reshape(N, 3, S)))
+    for beta in (0.0, 2e-6):
+        dt, lam = MembraneSystem.critical_dt(stand_in, beta=beta, iters=200)
+        assert 0.995 * lam_exact <= lam <= lam_exact * (1 + 1e-9)
+        # u+ = 2u - up - dt^2 lam (u + (beta/dt)(u - up)): roots of
+        # z^2 - (2 - dt^2 lam - dt beta lam) z + (1 - dt beta lam)
+        for f, stable in ((0.99, True), (1.01, False)):
+            h_ = f * dt
+            z = np.roots([1.0, -(2.0 - h_ * h_ * lam - h_ * beta * lam), 1.0 - h_ * beta * lam])
+            assert (np.max(np.abs(z)) <= 1.0 + 1e-12) == stable, (beta, f, np.abs(z))
