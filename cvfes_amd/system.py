@@ -508,7 +508,10 @@ class MembraneSystem:
 
         The central-difference recurrence with backward-difference stiffness damping is stable for
         ``dt^2 lam + 2 dt beta lam < 4``, i.e. ``dt < -beta + sqrt(beta^2 + 4/lam)``.  The full
-        operator (fixed rows included) bounds the constrained one from above, so the bound is safe.
+        operator (fixed rows included) bounds the constrained one from above, but the Rayleigh
+        quotient approaches ``lam_max`` from below, so the returned step can exceed the true limit
+        by the iteration's error (a few tenths of a percent after 60 iterations on the tubes):
+        run at a fraction of it.  Returns ``(dt_max, lam_max)``.
         """
         g = torch.Generator(device="cpu").manual_seed(seed)
         x = torch.rand(self.state_shape(), generator=g, dtype=torch.float64).to(self.device)
