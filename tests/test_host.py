@@ -557,3 +557,30 @@ def test_critical_dt_power_iteration_and_bound():
             h_ = f * dt
             z = np.roots([1.0, -(2.0 - h_ * h_ * lam - h_ * beta * lam), 1.0 - h_ * beta * lam])
             assert (np.max(np.abs(z)) <= 1.0 + 1e-12) == stable, (beta, f, np.abs(z))
+
+
+def test_load_tables_step_offsets_and_validation():
+    """``LoadTables.struct(k)`` points every per-step table at entry k (the fields stay whole),
+    and inconsistent tables are refused (CPU tensors; no GPU call)."""
+    import torch
+    from cvfes_amd.system import LoadTables
+    cpu = torch.device("cpu")
+    p = np.linspace(1.0, 2.0, 6)
+    g = np.arange(12.0).reshape(4, 3)
+    t = LoadTables(p, cpu, ga=g, qa=np.arange(6.0), gb=2 * g, qb=np.ones(6))
+    assert t.n == 6 and t.ga.shape == (4, 3) and t.qb.dtype == torch.float64
+    for k in (0, 3, 6):
+        s = t.struct(k)
+        assert s.p == t.p.data_ptr() + 8 * k
+        assert s.qa == t.qa.data_ptr() + 8 * k and s.qb == t.qb.data_ptr() + 8 * k
+        assert s.ga == t.ga.data_ptr() and s.gb == t.gb.data_ptr()
+    plain = LoadTables(3.5, cpu).struct(0)
+    assert plain.ga is None and plain.qa is None and plain.gb is None and plain.qb is None
+    with pytest.raises(ValueError):
+        t.struct(7)
+    with pytest.raises(ValueError):
+        t.struct(-1)
+    with pytest.raises(ValueError):
+        LoadTables(p, cpu, ga=g, qa=np.arange(5.0))            # a table one entry short
+    with pytest.raises(ValueError):
+        LoadTables(p, cpu, ga=g)                                # a field without its table
