@@ -584,3 +584,30 @@ def test_load_tables_step_offsets_and_validation():
         LoadTables(p, cpu, ga=g, qa=np.arange(5.0))            # a table one entry short
     with pytest.raises(ValueError):
         LoadTables(p, cpu, ga=g)                                # a field without its table
+
+
+def test_single_rank_contract_helpers():
+    """The reference helpers the drop-in keeps for one rank: boundary values, the union of the
+    local rows into the global ``[ndof, nSmp]`` array, no-op synchronisation (multi-rank
+    decomposition is refused), ``Steady`` false, and the one-rank communicator."""
+    from cvfes_amd.physicsSolver import SerialComm
+    from cvfes_amd.physicsSolverSDampGPUs import GPUSolidSolver
+
+    s = object.__new__(GPUSolidSolver)
+    s.size, s.ndof, s.nSmp = 1, 12, 2
+    s.dofs = np.array([3, 4, 5, 0, 1, 2, 9, 10, 11, 6, 7, 8])        # local rows -> global dofs
+    s.bdyDofs = np.array([0, 1, 2])
+    s.mesh = type("M", (), {"bdyU": 0.25})()
+    q = np.arange(24.0).reshape(12, 2)
+    s.ApplyBoundaryCondition(q)
+    assert np.all(q[:3] == 0.25) and q[3, 0] == 6.0
+    u = s.UnionDisplacement(q)
+    assert u.shape == (12, 2) and np.array_equal(u[s.dofs], q)
+    assert s.SyncCommNodes(q) is None and s.Steady() is False
+    s.size = 2
+    with pytest.raises(NotImplementedError):
+        s.SyncCommNodes(q)
+    with pytest.raises(NotImplementedError):
+        s.UnionDisplacement(q)
+    comm = SerialComm()
+    assert (comm.Get_size(), comm.Get_rank(), comm.Barrier(), comm.bcast("x")) == (1, 0, None, "x")
