@@ -90,6 +90,8 @@ class BatchedEnsemble:
         check(self.lib.cvfes_batch_plan(self.stream, arr, n, self.dev_meshes.data_ptr(),
                                         ctypes.byref(n_tasks), ctypes.byref(kinds)))
         self.host_meshes = arr
+        # the kernels read load-table entry (absolute step) with no bound of their own
+        self.max_steps = min(int(ld.n) for ld in self.loads)
         self.n_tasks = int(n_tasks.value)
         self.kinds = int(kinds.value)
         self._drop_graphs()
@@ -109,7 +111,14 @@ class BatchedEnsemble:
             pass
 
     # ------------------------------------------------------------------------------------------
+    def _check_steps(self, n):
+        """Refuse ``n`` more steps when they would run past any mesh's load tables."""
+        if n < 0 or self.step_index + n > self.max_steps:
+            raise ValueError(f"steps {self.step_index}..{self.step_index + n - 1} exceed the load "
+                             f"tables ({self.max_steps} entries in the shortest)")
+
     def _eager(self, n):
+        self._check_steps(n)
         for _ in range(n):
             check(self.lib.cvfes_batch_step(self.stream, self.dev_meshes.data_ptr(),
                                             len(self.systems), self.n_tasks, self.phase, None,
@@ -130,6 +139,7 @@ class BatchedEnsemble:
 
     def run(self, n, graph=True, block=99):
         """Advance every mesh by ``n`` steps."""
+        self._check_steps(n)
         if graph and n >= block:
             if block % 3:
                 raise ValueError("graph block must be a multiple of 3")
@@ -163,6 +173,10 @@ class BatchedEnsemble:
                  occupancies=(0, 6, 5, 4)):
         """Pick the lane width, register cap and grid size (wavefronts) with the shortest step,
         then the cap on the waves per SIMD for that choice."""
+        n_grid = len(candidates) if candidates is not None else 3
+        n_occ = sum(1 for o in occupancies if o != 0)
+        # every timed candidate runs 1 + steps eager steps of the ensemble's load tables
+        self._check_steps((len(lane_widths) * len(caps) * n_grid + n_occ) * (steps + 1))
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         best = None
         saved = (self.phase, self.step_index)

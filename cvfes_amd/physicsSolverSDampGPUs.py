@@ -20,6 +20,8 @@ kernels it replaces:
   stiffness and the pressure load are those of the moved geometry ``X + u``, re-formed every ``k``
   steps as the reference CPU path does (``physicsSolver.py:294-324``); ``k = 1`` is the
   every-step ``K(X+u)`` of ``assemble_K_P``.  This mode is matrix-free;
+* the start ``u_{-1} = u_0 - dt du_0 + dt^2/2 ddu_0`` takes ``ddu_0 = (f_0 - K u_0)/LHS`` without
+  the damping force, as ``calc_ddu`` does (``physicsSolverSDampGPUs.py:232-277``);
 * ``CalculateStress`` / ``SaveStress`` give the per-element stresses of
   ``SolidSolver.Save`` (``physicsSolver.py:384-424``) from a GPU kernel.
 """
@@ -146,8 +148,11 @@ class GPUSolidSolver(PhysicsSolver):
             self.tuned = self.system.autotune_occupancy(tune, self.dt, beta=self.beta, damp=self.damp,
                                                         method=self.method, steps=6)
         load0 = self._load_tables(np.array([self.time0]))
+        # calc_ddu forms ddu_0 = (P_0 - K u_0) / LHS with no damping force (physicsSolverSDampGPUs.py
+        # :232-246, .cl:398-409), so damp enters only the steps; beta (an extension, 0 unless set)
+        # adds beta K du_0 to the stiffness force of the start
         self.system.initial_state(self.u, self.du, load0[0][1], self.dt, beta=self.beta,
-                                  damp=self.damp, method=self.method)
+                                  damp=0.0, method=self.method)
         self.pressure = float(self.waveform(self.time0))
         self._step_load = None
         self._solve_tables = None

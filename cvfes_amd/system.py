@@ -32,7 +32,7 @@ def _dev(a, dtype, device):
 class LoadTables:
     """Device load tables ``f_n = p[n] f1 + qa[n] ga + qb[n] gb`` for a run of steps."""
 
-    def __init__(self, p, device, ga=None, qa=None, gb=None, qb=None):
+    def __init__(self, p, device, ga=None, qa=None, gb=None, qb=None, n_nodes=None):
         f64 = torch.float64
         self.p = p.to(device, f64).contiguous() if isinstance(p, torch.Tensor) else _dev(np.atleast_1d(np.asarray(p, np.float64)), f64, device)
         self.n = int(self.p.numel())
@@ -45,6 +45,11 @@ class LoadTables:
                 raise ValueError("load tables must all have one entry per step")
         if (self.ga is None) != (self.qa is None) or (self.gb is None) != (self.qb is None):
             raise ValueError("a traction field needs its table and vice versa")
+        # every step kernel reads g[3*row + i] for all rows: a field of another mesh would be read
+        # out of bounds on the device
+        for g in (self.ga, self.gb):
+            if g is not None and n_nodes is not None and tuple(g.shape) != (int(n_nodes), 3):
+                raise ValueError(f"traction field of shape {tuple(g.shape)}, expected ({n_nodes}, 3)")
 
     @staticmethod
     def _field(g, device):
@@ -436,7 +441,7 @@ class MembraneSystem:
 
     def loads(self, p, ga=None, qa=None, gb=None, qb=None):
         """Upload load tables (see :class:`LoadTables`)."""
-        return LoadTables(p, self.device, ga=ga, qa=qa, gb=gb, qb=qb)
+        return LoadTables(p, self.device, ga=ga, qa=qa, gb=gb, qb=qb, n_nodes=self.n_nodes)
 
     def initial_state(self, u0, du0, load, dt, beta=0.0, damp=0.0, method="stored", offset=0):
         """Allocate the three state buffers and set ``u_0``, ``u_{-1}`` (``u0``/``du0`` are ``[3N]``).
@@ -503,15 +508,19 @@ class MembraneSystem:
             lm = np.repeat(lm[:, None], self.n_smp, axis=1)
         return np.repeat(lm, 3, axis=0)
 
-    def critical_dt(self, beta=0.0, iters=60, seed=0, method="stored"):
+    def critical_dt(self, beta=0.0, iters=60, seed=0, method="stored", damp=0.0):
         """Largest stable step from a power iteration on ``M^-1 K`` (max over samples).
 
-        The central-difference recurrence with backward-difference stiffness damping is stable for
-        ``dt^2 lam + 2 dt beta lam < 4``, i.e. ``dt < -beta + sqrt(beta^2 + 4/lam)``.  The full
-        operator (fixed rows included) bounds the constrained one from above, but the Rayleigh
-        quotient approaches ``lam_max`` from below, so the returned step can exceed the true limit
-        by the iteration's error (a few tenths of a percent after 60 iterations on the tubes):
-        run at a fraction of it.  Returns ``(dt_max, lam_max)``.
+        A mode of ``M^-1 K`` with eigenvalue ``lam`` and mass-independent damping rate
+        ``g = damp / m`` obeys ``z^2 - (2 - dt^2 lam - c) z + (1 - c) = 0`` with
+        ``c = dt (beta lam + g)`` (backward-difference damping forces), whose roots stay in the
+        unit circle for ``dt^2 lam + 2 dt (beta lam + g) < 4``.  The bound is taken with ``lam``
+        and ``g`` both at their maxima (``g`` at the smallest non-zero mass), which for
+        ``damp = 0`` is ``dt < -beta + sqrt(beta^2 + 4/lam)``.  The full operator (fixed rows
+        included) bounds the constrained one from above, but the Rayleigh quotient approaches
+        ``lam_max`` from below, so the returned step can exceed the true limit by the iteration's
+        error (a few tenths of a percent after 60 iterations on the tubes): run at a fraction of
+        it.  Returns ``(dt_max, lam_max)``.
         """
         g = torch.Generator(device="cpu").manual_seed(seed)
         x = torch.rand(self.state_shape(), generator=g, dtype=torch.float64).to(self.device)
@@ -525,4 +534,9 @@ class MembraneSystem:
             y = kx / lm_b
             x = y / torch.linalg.vector_norm(y, dim=(0, 1))
         lam_max = float(lam.max())
-        return float(-beta + np.sqrt(beta * beta + 4.0 / lam_max)), lam_max
+        g = 0.0
+        if damp != 0.0:
+            m = lm[lm > 0]
+            g = float(damp) / float(m.min())
+        b = beta * lam_max + g
+        return float((-b + np.sqrt(b * b + 4.0 * lam_max)) / lam_max), lam_max

@@ -548,15 +548,20 @@ def test_critical_dt_power_iteration_and_bound():
         state_shape=lambda: (N, 3, S),
         spmv=lambda x, method: torch.as_tensor(so.multiply_by_vector(
             indptr, indices, K, x.reshape(3 * N, S).numpy()).reshape(N, 3, S)))
-    for beta in (0.0, 2e-6):
-        dt, lam = MembraneSystem.critical_dt(stand_in, beta=beta, iters=200)
+    m_min = float(LM[LM > 0].min())
+    for beta, damp in ((0.0, 0.0), (2e-6, 0.0), (0.0, 40.0), (2e-6, 40.0)):
+        dt, lam = MembraneSystem.critical_dt(stand_in, beta=beta, iters=200, damp=damp)
         assert 0.995 * lam_exact <= lam <= lam_exact * (1 + 1e-9)
-        # u+ = 2u - up - dt^2 lam (u + (beta/dt)(u - up)): roots of
-        # z^2 - (2 - dt^2 lam - dt beta lam) z + (1 - dt beta lam)
+        g = damp / m_min
+        # u+ = 2u - up - dt^2 lam (u + (beta/dt)(u - up)) - dt g (u - up): roots of
+        # z^2 - (2 - dt^2 lam - c) z + (1 - c),  c = dt (beta lam + g)
         for f, stable in ((0.99, True), (1.01, False)):
             h_ = f * dt
-            z = np.roots([1.0, -(2.0 - h_ * h_ * lam - h_ * beta * lam), 1.0 - h_ * beta * lam])
-            assert (np.max(np.abs(z)) <= 1.0 + 1e-12) == stable, (beta, f, np.abs(z))
+            c = h_ * (beta * lam + g)
+            z = np.roots([1.0, -(2.0 - h_ * h_ * lam - c), 1.0 - c])
+            assert (np.max(np.abs(z)) <= 1.0 + 1e-12) == stable, (beta, damp, f, np.abs(z))
+        if damp:
+            assert dt < MembraneSystem.critical_dt(stand_in, beta=beta, iters=200)[0]
 
 
 def test_load_tables_step_offsets_and_validation():
@@ -584,6 +589,12 @@ def test_load_tables_step_offsets_and_validation():
         LoadTables(p, cpu, ga=g, qa=np.arange(5.0))            # a table one entry short
     with pytest.raises(ValueError):
         LoadTables(p, cpu, ga=g)                                # a field without its table
+    # fields are checked against the mesh's node count (MembraneSystem.loads passes it)
+    assert LoadTables(p, cpu, ga=g, qa=np.arange(6.0), n_nodes=4).ga.shape == (4, 3)
+    with pytest.raises(ValueError, match="expected"):
+        LoadTables(p, cpu, ga=g, qa=np.arange(6.0), n_nodes=5)  # a field of another mesh
+    with pytest.raises(ValueError, match="expected"):
+        LoadTables(p, cpu, gb=np.arange(15.0), qb=np.ones(6), n_nodes=4)
 
 
 def test_single_rank_contract_helpers():
@@ -611,3 +622,65 @@ def test_single_rank_contract_helpers():
         s.UnionDisplacement(q)
     comm = SerialComm()
     assert (comm.Get_size(), comm.Get_rank(), comm.Barrier(), comm.bcast("x")) == (1, 0, None, "x")
+
+
+def test_batch_refuses_steps_past_load_tables():
+    """The batched kernel reads load-table entry (absolute step) with no bound of its own, so
+    ``BatchedEnsemble`` checks every run, eager block and tuning sweep against the shortest
+    mesh's tables before it launches anything (stand-in library and CPU buffers; no GPU call)."""
+    import types
+    import torch
+    from cvfes_amd import batch as B
+    from cvfes_amd._lib import CvfesSystem
+    from cvfes_amd.system import LoadTables
+
+    launched = []
+
+    class Lib:
+        def cvfes_batch_plan_bytes(self, n):
+            return 64 * n
+
+        def cvfes_batch_plan(self, stream, arr, n, dev, n_tasks, kinds):
+            n_tasks._obj.value, kinds._obj.value = 8 * n, 1
+            return 0
+
+        def cvfes_batch_step(self, *args):
+            launched.append(args[4])
+            return 0
+
+        def cvfes_graph_destroy(self, g):
+            return 0
+
+    class CpuEnsemble(B.BatchedEnsemble):
+        stream = None
+
+    cpu = torch.device("cpu")
+
+    def mesh(n_nodes):
+        bufs = [torch.zeros(n_nodes, 3, 2, dtype=torch.float64) for _ in range(3)]
+        return types.SimpleNamespace(n_nodes=n_nodes, state=bufs, _make_struct=CvfesSystem)
+
+    tables = [LoadTables(np.ones(9), cpu), LoadTables(np.ones(6), cpu)]
+    ens = object.__new__(CpuEnsemble)
+    ens.systems, ens.loads = [mesh(5), mesh(7)], tables
+    ens.dts, ens.methods, ens.betas, ens.damps = [1e-6] * 2, ["matfree"] * 2, [0.0] * 2, [0.0] * 2
+    ens.device, ens.lib, ens._graphs = cpu, Lib(), {}
+    ens.vw_hint, ens.cap, ens.waves_per_simd, ens.n_waves = 0, False, 0, 0
+    ens._plan()
+    ens.phase, ens.step_index = 0, 0
+    assert ens.max_steps == 6 and ens.n_tasks == 16
+    with pytest.raises(ValueError, match="exceed the load tables"):
+        ens.run(7, graph=False)
+    assert launched == [] and ens.step_index == 0
+    ens.run(4, graph=False)
+    assert launched == [0, 1, 2, 0] and ens.step_index == 4
+    with pytest.raises(ValueError, match="exceed the load tables"):
+        ens._eager(3)
+    with pytest.raises(ValueError, match="exceed the load tables"):
+        ens.run(99, graph=True)
+    with pytest.raises(ValueError, match="exceed the load tables"):
+        ens.autotune(steps=2, lane_widths=(0,), caps=(False,), occupancies=(0,),
+                     candidates=[64])                       # 1 candidate x (1 + 2) steps > 2 left
+    assert launched == [0, 1, 2, 0] and ens.step_index == 4
+    ens.run(2, graph=False)
+    assert ens.step_index == 6
