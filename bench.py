@@ -92,8 +92,12 @@ def time_steps(sysm, load, dt, beta, method, warmup, steps):
     return wall, kern_ms
 
 
-def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads):
-    """C/OpenMP restatement of the reference CPU step on a bounded sample; returns (result, u_cpu)."""
+def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads, warmup=2,
+                 one_thread_steps=3):
+    """C/OpenMP restatement of the reference CPU step on a bounded sample; returns (result, u_cpu).
+
+    ``steps`` steps of the recurrence on ``threads`` threads, the first ``warmup`` untimed, then
+    ``one_thread_steps`` more steps of the same recurrence on one thread."""
     from oracle import cpu_oracle as co
     from oracle import solid_oracle as so
     co.build()
@@ -103,7 +107,7 @@ def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads):
     t_asm = time.perf_counter() - t_asm
     LM = sysm.lumped_mass_dofs()
     f1 = sysm.f1.cpu().numpy().ravel()
-    p = case.pressures(0, steps + 1, dt)
+    p = case.pressures(0, steps + one_thread_steps, dt)
     ndof = 3 * sysm.n_nodes
     # u0 = 0 and du0 = 0: K u0 = 0, so u_-1 = dt^2/2 * f_0 / LM (physicsSolver.py:217-225).
     u = np.zeros((ndof, S))
@@ -118,20 +122,25 @@ def cpu_baseline(case, sysm, nodes, el, bdy, coefK, dt, beta, steps, threads):
                        bdy_dofs=bdy_dofs, threads=threads)
         times.append(time.perf_counter() - t0)
         up, u = u, unew
-    timed = times[1:] if len(times) > 1 else times
+    timed = times[warmup:] if len(times) > warmup else times
     rate = S / float(np.mean(timed))
     res = {"value": rate, "unit": "sample-steps/s", "cores": int(threads),
            "kind": "port",
            "sample": f"{case.name}: same mesh, {S} samples and inputs; {len(timed)} timed steps after "
-                     f"1 warm-up ({float(np.sum(times)):.1f} s of CPU work); K assembled in the "
-                     f"reference layout [nnzb][S][3][3] in {t_asm:.1f} s (not timed)"}
-    if threads != 1:
-        # the same step on one thread (the next step of the recurrence, result discarded)
-        t0 = time.perf_counter()
-        co.step(sysm.indptr_h, sysm.indices_h, K, LM, LHS, f1, p[steps], u, up, dt, beta=beta,
-                bdy_dofs=bdy_dofs, threads=1)
-        t1 = time.perf_counter() - t0
-        res["one_thread"] = {"value": S / t1, "unit": "sample-steps/s", "cores": 1, "steps": 1}
+                     f"{len(times) - len(timed)} warm-up ({float(np.sum(times)):.1f} s of CPU work); "
+                     f"K assembled in the reference layout [nnzb][S][3][3] in {t_asm:.1f} s (not timed)",
+           "step_s": [float(t) for t in timed]}
+    if threads != 1 and one_thread_steps > 0:
+        # the next steps of the recurrence on one thread (results discarded)
+        a, b, t1 = u, up, []
+        for n in range(steps, steps + one_thread_steps):
+            t0 = time.perf_counter()
+            c = co.step(sysm.indptr_h, sysm.indices_h, K, LM, LHS, f1, p[n], a, b, dt, beta=beta,
+                        bdy_dofs=bdy_dofs, threads=1)
+            t1.append(time.perf_counter() - t0)
+            a, b = c, a
+        res["one_thread"] = {"value": S / float(np.mean(t1)), "unit": "sample-steps/s", "cores": 1,
+                             "steps": len(t1), "step_s": [float(t) for t in t1]}
     return res, u
 
 
@@ -153,7 +162,8 @@ def main():
     ap.add_argument("--row-order", default="auto", choices=["auto", "natural", "strips"],
                     help="row visiting order of the timed step (auto: the fastest of natural and "
                          "strips 32/64 on 20 steps each; the result does not depend on it)")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="CPU baseline steps (first is warm-up)")
+    ap.add_argument("--cpu-steps", type=int, default=22,
+                    help="CPU baseline steps (the first two are warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matfree", action="store_true")
@@ -235,6 +245,13 @@ def main():
                      "algorithmic_bytes_per_launch": abytes, "kernel_ms": kern_ms,
                      "kernel": "k_step (fused block-CSR product + central-difference update)"},
     }
+    # achievable streaming rate of this box with the step kernels' load forms (calibration kernels)
+    ceil = stream_ceiling()
+    out["stream_ceiling"] = ceil
+    out["roofline"]["frac_of_stream_read_nt"] = achieved / ceil["read_nt_GBs"]
+    print(f"[bench] streaming ceiling: {ceil['read_nt_GBs']:.0f} GB/s non-temporal reads, "
+          f"{ceil['read_GBs']:.0f} GB/s reads, {ceil['copy_nt_GBs']:.0f} GB/s copy",
+          file=sys.stderr, flush=True)
     if not args.no_matfree:
         # the other operator and the symmetric storage of the stored one, measured the same way
         alts = [("matfree", "auto", "natural")] if args.method == "stored" else [("stored", "auto", "natural")]
@@ -285,31 +302,61 @@ def main():
 
 
 
-def run_copy(args):
-    """Calibration: ``dst = src * 1`` over ``--copy-bytes`` (reads and writes that many bytes)."""
+def stream_ceiling(nbytes=4 << 30, reps=5):
+    """Streaming rates of the hand-written calibration kernels (``cvfes_stream_read`` / ``_copy``,
+    the step kernels' 16-byte load forms) over ``nbytes``, timed with HIP events on the launch
+    stream: GB/s for non-temporal and ordinary reads and for a copy (read + write counted)."""
     import torch
+    from cvfes_amd._lib import check, lib
+    h = lib()
+    n = (nbytes // 16) * 2
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    part = torch.empty(int(h.cvfes_stream_blocks(n)), dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    sh = st.cuda_stream
+    out = {"bytes": 8 * n}
+    for name, fn, nt, moved in (("read_nt", h.cvfes_stream_read, 1, 8 * n),
+                                ("read", h.cvfes_stream_read, 0, 8 * n),
+                                ("copy_nt", h.cvfes_stream_copy, 1, 16 * n)):
+        target = part if fn is h.cvfes_stream_read else dst
+        for _ in range(2):
+            check(fn(sh, src.data_ptr(), n, nt, target.data_ptr()))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            check(fn(sh, src.data_ptr(), n, nt, target.data_ptr()))
+        e1.record(st)
+        torch.cuda.synchronize()
+        out[f"{name}_GBs"] = moved / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+    if abs(float(part.sum()) - n) > 1e-6 * n or not bool(torch.equal(dst, src)):
+        raise RuntimeError("streaming calibration kernels returned wrong results")
+    del src, dst, part
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_copy(args):
+    """Calibration: the hand-written streaming kernels over ``--copy-bytes`` (see
+    :func:`stream_ceiling`), and torch's ``dst = src * 1`` for comparison."""
+    import torch
+    res = {"workload": "copy"}
+    res.update(stream_ceiling(args.copy_bytes))
     n = args.copy_bytes // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(5):
         torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    # the same launches timed with events (copy: read + write), and a read-only stream (sum)
     st = torch.cuda.current_stream()
-    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
     e0.record(st)
     for _ in range(5):
         torch.mul(src, 1.0, out=dst)
     e1.record(st)
-    for _ in range(5):
-        src.sum()
-    e2.record(st)
     torch.cuda.synchronize()
-    t_copy = e0.elapsed_time(e1) / 5e3
-    t_sum = e1.elapsed_time(e2) / 5e3
-    print(json.dumps({"workload": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 5,
-                      "copy_GBps": 16 * n / t_copy / 1e9, "read_sum_GBps": 8 * n / t_sum / 1e9}),
-          flush=True)
+    res["torch_mul_copy_GBs"] = 16 * n / (e0.elapsed_time(e1) / 5e3) / 1e9
+    print(json.dumps(res), flush=True)
 
 
 def run_batch(args):

@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -89,6 +89,9 @@ _SIGNATURES = {
     "cvfes_last_error": (ctypes.c_char_p, []),
     "cvfes_stream_sync": (_int, [_vp]),
     "cvfes_struct_size": (_i64, [_int]),
+    "cvfes_stream_blocks": (_i64, [_i64]),
+    "cvfes_stream_read": (_int, [_vp, _vp, _i64, _int, _vp]),
+    "cvfes_stream_copy": (_int, [_vp, _vp, _i64, _int, _vp]),
     "cvfes_element_matrices": (_int, [_vp, _i64, _vp, _vp, _f64, _f64, _vp, _vp, _vp, _vp]),
     "cvfes_element_coef": (_int, [_vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _int, _vp]),
     "cvfes_lumped_mass_load": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,

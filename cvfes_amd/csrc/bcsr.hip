@@ -782,16 +782,13 @@ int validate_load(const cvfes_load_t *load, const char *who) {
     return CVFES_OK;
 }
 
-extern "C" int cvfes_initial_state(void *stream, const cvfes_system_t *sys, int method,
-                                   const double *u0, const double *du0, const cvfes_load_t *load,
-                                   double dt, double beta, double damp, double *u, double *up,
-                                   double *work) {
-    if (int rc = validate(sys, method, "cvfes_initial_state")) return rc;
-    if (int rc = validate_load(load, "cvfes_initial_state")) return rc;
-    CVFES_REQUIRE(u0 && du0 && u && up && work && sys->lm && sys->f1,
-                  "cvfes_initial_state: null pointer");
-    CVFES_REQUIRE(dt > 0.0, "cvfes_initial_state: dt must be positive");
-    hipStream_t st = static_cast<hipStream_t>(stream);
+namespace {
+
+// The two launches of cvfes_initial_state around the product K (u0 + beta du0), which `work`
+// ([N][3][S]) holds between them.
+int initial_state_impl(hipStream_t st, const cvfes_system_t *sys, int method, const double *u0,
+                       const double *du0, const cvfes_load_t *load, double dt, double beta,
+                       double damp, double *u, double *up, double *work) {
     const int64_t ndof = 3 * sys->n_nodes, S = sys->n_smp;
     hipLaunchKernelGGL(k_tile_combo, dim3(grid_for(ndof * S)), dim3(kBlock), 0, st, ndof, S, u0,
                        du0, beta, u);
@@ -805,6 +802,30 @@ extern "C" int cvfes_initial_state(void *stream, const cvfes_system_t *sys, int 
     hipLaunchKernelGGL(k_init_finish, dim3(grid_for(ndof * S)), dim3(kBlock), 0, st, ndof, S, u0,
                        du0, sys->f1, *load, damp, dt, sys->lm, sys->lm_sstride, work, u, up);
     return check_launch("k_init_finish");
+}
+
+int validate_initial_state(const cvfes_system_t *sys, int method, const cvfes_load_t *load,
+                           const double *u0, const double *du0, double dt, const double *u,
+                           const double *up, const char *who) {
+    if (int rc = validate(sys, method, who)) return rc;
+    if (int rc = validate_load(load, who)) return rc;
+    CVFES_REQUIRE(u0 && du0 && u && up && sys->lm && sys->f1, "%s: null pointer", who);
+    CVFES_REQUIRE(dt > 0.0, "%s: dt must be positive", who);
+    return CVFES_OK;
+}
+
+}  // namespace
+
+extern "C" int cvfes_initial_state(void *stream, const cvfes_system_t *sys, int method,
+                                   const double *u0, const double *du0, const cvfes_load_t *load,
+                                   double dt, double beta, double damp, double *u, double *up,
+                                   double *work) {
+    if (int rc = validate_initial_state(sys, method, load, u0, du0, dt, u, up,
+                                        "cvfes_initial_state"))
+        return rc;
+    CVFES_REQUIRE(work, "cvfes_initial_state: null work buffer");
+    return initial_state_impl(static_cast<hipStream_t>(stream), sys, method, u0, du0, load, dt,
+                              beta, damp, u, up, work);
 }
 
 extern "C" int cvfes_step(void *stream, const cvfes_system_t *sys, int method, double **state,

@@ -1,7 +1,8 @@
 """Solid-only run driver with the ``CVFES`` call sequence (``cvfes.py:26-119``, ``main.py:21-49``).
 
-``CVFES(comm).ReadInputFile(cfg, nSmp)`` builds the wall mesh either from a VTK ``.vtp`` file with
-``faces`` (boundary = union of the face nodes, ``mesh.py:175-184,658-665``) or, with
+``CVFES(comm).ReadInputFile(cfg, nSmp)`` builds the wall mesh either from a VTK ``.vtp`` or ``.vtu``
+file with ``faces`` (boundary = union of the face nodes matched by ``GlobalNodeID``,
+``mesh.py:50-67,175-184,658-665``) or, with
 ``generator = tube`` in the wall mesh section, from a synthetic tube.  Young's modulus and thickness
 come from the solid equation section: uniform values, ``.npy`` property files (``mesh.py:620-655``)
 or, with ``Youngs Modulus model = gmrf_element``, per-element GMRF samples.  ``Solve()`` runs
@@ -41,20 +42,24 @@ def build_wall_mesh(mc, eq, nSmp):
             profile = stenosed_radius(mc.radius, r_out, mc.length, severity=mc.stenosis)
         nodes, elements, boundary = tube_geometry(mc.ntheta, mc.nz, mc.radius, mc.length, profile)
     else:
-        wall = vtkio.read_vtp(mc.file_path)
+        wall = vtkio.read_vtk(mc.file_path)
         nodes, elements = wall.points, wall.polys
         gids = wall.point_data.get("GlobalNodeID")
         boundary = []
         for face in mc.faces:
             if face.name != "boundaries":
                 continue
+            if gids is None:
+                raise ValueError("wall mesh has no GlobalNodeID array to match its faces")
+            # first wall node carrying each face GlobalNodeID; ids absent from the wall are skipped
+            # (Face.adjustNodeIds, mesh.py:59-67)
+            lookup = {}
+            for i, g in enumerate(np.asarray(gids).ravel()):
+                lookup.setdefault(int(g), i)
             paths = face.file_path if isinstance(face.file_path, list) else [face.file_path]
             for p in paths:
-                fgids = vtkio.read_vtp(p).point_data["GlobalNodeID"]
-                if gids is None:
-                    raise ValueError("wall mesh has no GlobalNodeID array to match its faces")
-                lookup = {int(g): i for i, g in enumerate(gids)}
-                boundary.extend(lookup[int(g)] for g in fgids)
+                fgids = np.asarray(vtkio.read_vtk(p).point_data["GlobalNodeID"]).ravel()
+                boundary.extend(lookup[int(g)] for g in fgids if int(g) in lookup)
         boundary = np.array(sorted(set(boundary)), dtype=np.int64)
     n = nodes.shape[0]
     vthick = (np.full((n, nSmp), eq.thickness) if eq.thicknessFilename is None
@@ -68,9 +73,9 @@ def build_wall_mesh(mc, eq, nSmp):
     iniU = iniDu = None
     ic = eq.initialConditions
     if ic is not None and isinstance(ic.displacement, str):
-        iniU = vtkio.read_vtp(ic.displacement).point_data["displacement"].ravel()
+        iniU = np.asarray(vtkio.read_vtk(ic.displacement).point_data["displacement"], np.float64).ravel()
     if ic is not None and isinstance(ic.velocity, str):
-        iniDu = vtkio.read_vtp(ic.velocity).point_data["velocity"].ravel()
+        iniDu = np.asarray(vtkio.read_vtk(ic.velocity).point_data["velocity"], np.float64).ravel()
     if ic is not None and not isinstance(ic.displacement, str) and ic.displacement:
         iniU = np.full(3 * n, float(ic.displacement))
     if ic is not None and not isinstance(ic.velocity, str) and ic.velocity:

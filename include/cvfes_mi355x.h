@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 9
+#define CVFES_ABI_VERSION 10
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -140,6 +140,23 @@ const char *cvfes_last_error(void);
 
 /* Wait for all work on `stream`. */
 int cvfes_stream_sync(void *stream);
+
+/*
+ * Streaming calibration (no counterpart in the reference): the HBM rate reached with the step
+ * kernels' own load and store forms, used as the achievable ceiling next to the 8 TB/s peak and as
+ * the known byte count that scales the traffic counters.  Each lane moves 16-byte items (two
+ * doubles), four independent loads in flight; `nontemporal` selects non-temporal loads (the
+ * stored values' form) or ordinary ones; stores are 16-byte non-temporal (the step outputs' form).
+ *   cvfes_stream_read: partial[b] = sum of the workgroup's items, for b < cvfes_stream_blocks(n)
+ *     (reads 8 n bytes, writes 8 bytes per workgroup);
+ *   cvfes_stream_copy: dst = src (reads and writes 8 n bytes).
+ * n_doubles must be even and the buffers 16-byte aligned.
+ */
+int64_t cvfes_stream_blocks(int64_t n_doubles);
+int cvfes_stream_read(void *stream, const double *src, int64_t n_doubles, int nontemporal,
+                      double *partial);
+int cvfes_stream_copy(void *stream, const double *src, int64_t n_doubles, int nontemporal,
+                      double *dst);
 
 /*
  * Unit-coefficient element stiffness blocks, areas and normals.

@@ -35,6 +35,9 @@ def lib():
         h.oracle_spmv.argtypes = [_i64, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
         h.oracle_step.argtypes = [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp,
                                   _f64, _f64, _f64, _vp, _i64, _f64, _vp, ctypes.c_int]
+        h.oracle_run.argtypes = [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
+                                 _vp, _f64, _f64, _f64, _vp, _i64, _f64, _vp, _i64, _vp,
+                                 ctypes.c_int]
         h.oracle_max_threads.restype = ctypes.c_int
         _lib = h
     return _lib
@@ -82,6 +85,32 @@ def step(indptr, indices, K, LM, LHS, f1, p, u, up, dt, beta=0.0, damp=0.0, bdy_
     if rc != 0:
         raise MemoryError("oracle_step: allocation failed")
     return unew
+
+
+def run(indptr, indices, K, LM, LHS, f1, p, u, up, dt, beta=0.0, damp=0.0, bdy_dofs=None,
+        bdy_u=0.0, snap_steps=(), threads=0):
+    """``len(p)`` steps of :func:`step` (bit-identical results) in one native call.
+
+    Returns ``(u_n, u_{n-1}, snaps)`` with ``snaps[i]`` the state after ``snap_steps[i]`` steps.
+    """
+    S = K.shape[1]
+    n = indptr.size - 1
+    p = np.ascontiguousarray(np.atleast_1d(p), np.float64)
+    u = np.array(u, np.float64, order="C", copy=True)
+    up = np.array(up, np.float64, order="C", copy=True)
+    bd = np.ascontiguousarray(np.zeros(0, np.int64) if bdy_dofs is None else bdy_dofs, np.int64)
+    snap = np.ascontiguousarray(snap_steps, np.int64)
+    if snap.size and (snap.min() < 1 or snap.max() > p.size):
+        raise ValueError("snapshot steps must lie in 1..len(p)")
+    snaps = np.empty((snap.size, 3 * n, S))
+    rc = lib().oracle_run(n, S, indices.size, _p(indptr), _p(indices), _p(np.ascontiguousarray(K)),
+                          _p(np.ascontiguousarray(LM)), _p(np.ascontiguousarray(LHS)),
+                          _p(np.ascontiguousarray(f1, np.float64)), _p(p), p.size, _p(u), _p(up),
+                          float(dt), float(beta), float(damp), _p(bd), bd.size, float(bdy_u),
+                          _p(snap), snap.size, _p(snaps), threads)
+    if rc != 0:
+        raise MemoryError("oracle_run: allocation failed")
+    return u, up, snaps
 
 
 def max_threads():

@@ -212,4 +212,74 @@ int oracle_step(int64_t n_nodes, int64_t n_smp, int64_t nnzb, const int64_t *ind
     return 0;
 }
 
+/*
+ * n_steps steps of oracle_step with the pressure table p[n_steps], for long runs.  (dt^2) K is
+ * formed once instead of every step: it is the same deterministic product each time, and the
+ * vector passes of oracle_step are done per entry in the same order, so every step's result equals
+ * oracle_step's bit for bit.  On entry u = u_0, up = u_{-1}; on return u = u_n, up = u_{n-1}.
+ * After step snap_steps[i] (1-based count of completed steps) u is copied to snaps[i][ndof][S].
+ */
+int oracle_run(int64_t n_nodes, int64_t n_smp, int64_t nnzb, const int64_t *indptr,
+               const int64_t *indices, const double *K, const double *LM, const double *LHS,
+               const double *f1, const double *p, int64_t n_steps, double *u, double *up,
+               double dt, double beta, double damp, const int64_t *bdy_dofs, int64_t n_bdy,
+               double bdy_u, const int64_t *snap_steps, int64_t n_snap, double *snaps,
+               int n_threads) {
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    const int64_t ndof = 3 * n_nodes, n = ndof * n_smp, nk = nnzb * n_smp * 9;
+    const double dt2 = dt * dt;
+    double *Kdt = (double *)malloc(sizeof(double) * nk);
+    double *x = (double *)malloc(sizeof(double) * n);
+    double *Ku = (double *)malloc(sizeof(double) * n);
+    double *unew = (double *)malloc(sizeof(double) * n);
+    double *cur = (double *)malloc(sizeof(double) * n);
+    double *prev = (double *)malloc(sizeof(double) * n);
+    if (!Kdt || !x || !Ku || !unew || !cur || !prev) {
+        free(Kdt); free(x); free(Ku); free(unew); free(cur); free(prev);
+        return -1;
+    }
+    memcpy(cur, u, sizeof(double) * n);
+    memcpy(prev, up, sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nk; ++k) Kdt[k] = dt2 * K[k];
+    const double bdt = beta / dt;
+    for (int64_t step = 0; step < n_steps; ++step) {
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < n; ++k) {
+            x[k] = beta != 0.0 ? cur[k] + bdt * (cur[k] - prev[k]) : cur[k];
+            Ku[k] = 0.0;
+        }
+        multiply_by_vector(n_nodes, n_smp, indptr, indices, Kdt, x, Ku);
+        const double pn = p[step];
+#pragma omp parallel for schedule(static)
+        for (int64_t d = 0; d < ndof; ++d)
+            for (int64_t s = 0; s < n_smp; ++s) {
+                const int64_t k = d * n_smp + s;
+                double f = pn * f1[d];
+                if (damp != 0.0) f -= damp * (cur[k] - prev[k]) / dt;
+                double a1 = dt2 * f;
+                double a2 = 2.0 * LM[k];
+                a2 = a2 * cur[k];
+                a2 = a2 - Ku[k];
+                double a3 = 0.0 - LM[k];
+                a3 = a3 * prev[k];
+                a1 = a1 + a2;
+                a1 = a1 + a3;
+                unew[k] = LHS[k] != 0.0 ? a1 / LHS[k] : 0.0;
+            }
+        for (int64_t b = 0; b < n_bdy; ++b)
+            for (int64_t s = 0; s < n_smp; ++s) unew[bdy_dofs[b] * n_smp + s] = bdy_u;
+        double *t = prev;
+        prev = cur;
+        cur = unew;
+        unew = t;
+        for (int64_t i = 0; i < n_snap; ++i)
+            if (snap_steps[i] == step + 1) memcpy(snaps + i * n, cur, sizeof(double) * n);
+    }
+    memcpy(u, cur, sizeof(double) * n);
+    memcpy(up, prev, sizeof(double) * n);
+    free(Kdt); free(x); free(Ku); free(unew); free(cur); free(prev);
+    return 0;
+}
+
 int oracle_max_threads(void) { return omp_get_max_threads(); }

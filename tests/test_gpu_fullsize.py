@@ -7,10 +7,15 @@ the mathematics fixes, plus the C/OpenMP oracle at the 49,152-node size:
 * permuting the samples permutes the results exactly (samples are independent);
 * superposition: the response to p1 + p2 equals the sum of the responses (linear system, zero
   initial state);
-* the 49,152-node x 32-sample tube after 20 steps matches the C oracle (1e-10).
+* the 49,152-node x 32-sample tube after 20 steps matches the C oracle (1e-10);
+* the plain configuration itself (102,400 nodes, 128 samples, beta = 1e-4, dt = 1.2e-8 s) over its
+  full 2,000 steps: four seeded sample columns of the GPU run against the C oracle run on just those
+  columns (samples are independent), at steps 1, 500 and 2,000, stored and matrix-free (1e-10).
 """
 import numpy as np
 import pytest
+
+from oracle_cases import oracle_mass_load, oracle_operator, oracle_trajectory
 
 pytestmark = pytest.mark.gpu
 
@@ -76,7 +81,6 @@ def test_medium_tube_matches_c_oracle(require_gpu):
     from cvfes_amd.configs import get_case
     from cvfes_amd.system import MembraneSystem
     from oracle import cpu_oracle as co
-    from oracle import solid_oracle as so
     case = get_case("tube-192x256-s32")
     nodes, el, bdy = case.geometry()
     N, S = nodes.shape[0], case.n_smp
@@ -86,20 +90,9 @@ def test_medium_tube_matches_c_oracle(require_gpu):
     steps = 20
     p = case.pressures(0, steps)
     ug = run(sysm, p, case.dt, 0.0, "stored", steps).reshape(-1, S)
-    # oracle: reference-layout K assembled on the CPU from the same coefficients
-    _, gth = so.gauss_properties(np.full((N, S), case.E0), np.full((N, S), case.thickness), el)
-    coefK = so.coef_k_element_E(gth, E)
-    K = co.assemble_K(nodes, el, so.d_matrix(case.nu), coefK, sysm.indptr_h, sysm.indices_h)
-    # lumped mass and unit load from the oracle's element areas/normals (closed forms pinned in
-    # test_oracle: rho A h / 9 per incident element and A / 3 * normal)
-    _, area, normal = so.element_blocks_closed_form(nodes, el, case.nu)
-    lm_node = np.zeros(N)
-    f1 = np.zeros((N, 3))
-    for a in range(3):
-        np.add.at(lm_node, el[:, a], case.density * area * case.thickness / 9.0)
-        np.add.at(f1, el[:, a], (area / 3.0)[:, None] * normal)
-    LM = np.repeat(np.repeat(lm_node, 3)[:, None], S, axis=1)
-    f1 = f1.ravel()
+    K = oracle_operator(nodes, el, E, case.E0, case.thickness, case.nu, sysm.indptr_h,
+                        sysm.indices_h)
+    LM, f1 = oracle_mass_load(nodes, el, case.density, case.thickness, case.nu, S)
     u = np.zeros((3 * N, S))
     up = 0.5 * case.dt ** 2 * (p[0] * f1[:, None] / LM)
     bd = (3 * bdy[:, None] + np.arange(3)).ravel()
@@ -108,3 +101,39 @@ def test_medium_tube_matches_c_oracle(require_gpu):
                         bdy_dofs=bd), u
     assert np.max(np.abs(u)) > 0
     assert rel_err(ug, u) < 1e-10
+
+
+SNAP_STEPS = (1, 500, 2000)
+
+
+@pytest.fixture(scope="module")
+def plain_oracle_columns(plain):
+    """C oracle over the configured 2,000 steps on four seeded sample columns."""
+    case, sysm, (nodes, el, bdy, E) = plain
+    assert case.nsteps == SNAP_STEPS[-1]
+    cols = np.sort(np.random.default_rng(7).choice(case.n_smp, 4, replace=False))
+    p = case.pressures(0, case.nsteps)
+    snaps = oracle_trajectory(nodes, el, bdy, E[:, cols], p, case.dt, case.beta, case.E0,
+                              case.thickness, case.density, case.nu, sysm.indptr_h, sysm.indices_h,
+                              SNAP_STEPS)
+    return cols, p, snaps
+
+
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_plain_config_full_length_matches_c_oracle(plain, plain_oracle_columns, method):
+    case, sysm, _ = plain
+    cols, p, snaps = plain_oracle_columns
+    N = sysm.n_nodes
+    u0 = np.zeros(3 * N)
+    load = sysm.loads(p)
+    sysm.initial_state(u0, u0, load, case.dt, beta=case.beta, method=method)
+    done = 0
+    for i, k in enumerate(SNAP_STEPS):
+        sysm.step(load, case.dt, beta=case.beta, method=method, offset=done, nsteps=k - done)
+        done = k
+        ug = sysm.u[:, :, cols].cpu().numpy().reshape(3 * N, cols.size)
+        assert np.all(np.isfinite(ug))
+        err = rel_err(ug, snaps[i])
+        print(f"{method}: step {k}: max |u| {np.max(np.abs(snaps[i])):.3e}, rel err {err:.2e}")
+        assert np.max(np.abs(snaps[i])) > 0
+        assert err < 1e-10, (method, k, err)

@@ -55,17 +55,18 @@ class OracleModel:
     def __init__(self, mesh):
         el, n, S = mesh.elementNodeIds, mesh.nNodes, mesh.nSmp
         self.indptr, self.indices = so.sparse_info_loops(el, n)
-        _, gth = so.gauss_properties(mesh.vE, mesh.vthickness, el)
-        coefK = so.coef_k_element_E(gth, mesh.elmE)
+        gE, gth = so.gauss_properties(mesh.vE, mesh.vthickness, el)
+        coefK = so.coef_k(gth, gE) if mesh.elmE is None else so.coef_k_element_E(gth, mesh.elmE)
         _, self.K, self.f1, self.LM = so.assemble_reference(mesh.nodes, el, 1.0, so.d_matrix(mesh.v),
                                                             coefK, gth, mesh.density, self.indptr,
                                                             self.indices)
         self.bdy_dofs = (3 * mesh.boundary[:, None] + np.arange(3)).ravel()
 
-    def run(self, pressures, dt, beta=0.0, damp=0.0, **kw):
+    def run(self, pressures, dt, beta=0.0, damp=0.0, u0=None, du0=None, **kw):
         n = self.indptr.size - 1
-        u0 = np.zeros(3 * n)
-        u, _, _ = so.run(self.indptr, self.indices, self.K, self.LM, self.f1, pressures, u0, u0, dt,
+        u0 = np.zeros(3 * n) if u0 is None else u0
+        du0 = np.zeros(3 * n) if du0 is None else du0
+        u, _, _ = so.run(self.indptr, self.indices, self.K, self.LM, self.f1, pressures, u0, du0, dt,
                          beta=beta, damp=damp, bdy_dofs=self.bdy_dofs, **kw)
         return u
 
@@ -103,6 +104,26 @@ def test_fast_run_matches_oracle_exact_times(tmp_path, driver, method):
     p = wf(np.arange(n_steps) * DT)
     om = OracleModel(mesh)
     u_ref = om.run(p, DT, beta=2e-7, damp=20.0)
+    assert rel_err(solver.solidSolver.GetDisplacement(), u_ref) < REL
+
+
+@pytest.mark.parametrize("method", ["stored", "matfree"])
+def test_case_read_from_files_matches_oracle(tmp_path, driver, method):
+    """A case whose wall, faces, property fields and initial state come from files (VTK's default
+    appended/base64/zlib encoding, a .vtu face, .npy fields with the rho suffix) solved on the GPU."""
+    import vtkfiles as vf
+    cfg, want = vf.write_cvfes_case(str(tmp_path), nsteps=80, dt=DT, method=method)
+    cv = driver().ReadInputFile(cfg)
+    mesh = cv.meshes["wall"]
+    assert np.array_equal(mesh.vE, want["E"]) and np.array_equal(mesh.iniU, want["u0"])
+    solver = cv.Solve(fast=True)
+    n_steps = solver.nTimeSteps
+    assert n_steps == 80
+    p = solver.solidSolver.waveform(np.arange(n_steps) * DT)
+    om = OracleModel(mesh)
+    assert np.array_equal(np.sort(om.bdy_dofs // 3), np.repeat(want["boundary"], 3))
+    u_ref = om.run(p, DT, u0=mesh.iniU, du0=mesh.iniDu)
+    assert np.max(np.abs(u_ref)) > 0
     assert rel_err(solver.solidSolver.GetDisplacement(), u_ref) < REL
 
 

@@ -461,7 +461,7 @@ def test_bench_cpu_baseline_is_the_reference_step():
     coefK = so.coef_k_element_E(gth, elmE)
     _, K, f1, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(nu), coefK, gth, 1.0,
                                          indptr, indices)
-    pres = 1.33e4 * (1.0 + 0.1 * np.sin(np.arange(steps + 1)))
+    pres = 1.33e4 * (1.0 + 0.1 * np.sin(np.arange(steps + 3)))
     sysm = types.SimpleNamespace(indptr_h=indptr, indices_h=indices, n_nodes=N,
                                  f1=torch.as_tensor(f1), lumped_mass_dofs=lambda: LM)
     case = types.SimpleNamespace(n_smp=S, name="tube-16x6", nu=nu,
@@ -475,6 +475,9 @@ def test_bench_cpu_baseline_is_the_reference_step():
     assert np.max(np.abs(u - uo)) / np.max(np.abs(uo)) < 1e-12
     assert res["kind"] == "port" and res["cores"] == 2 and res["value"] > 0
     assert res["one_thread"]["cores"] == 1 and res["one_thread"]["value"] > 0
+    # two warm-up steps, the rest timed; three more steps of the recurrence on one thread
+    assert len(res["step_s"]) == steps - 2 and "3 timed steps after 2 warm-up" in res["sample"]
+    assert res["one_thread"]["steps"] == 3
 
 
 def test_solver_reports_a_non_finite_displacement():

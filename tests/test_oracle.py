@@ -381,6 +381,53 @@ def test_c_oracle_matches_numpy_oracle():
     assert np.allclose(a, b, rtol=0, atol=1e-14 * np.abs(a).max())
 
 
+def test_c_oracle_run_equals_repeated_steps():
+    """The long-run form equals step-by-step calls bit for bit, snapshots included."""
+    from oracle import cpu_oracle as co
+    nodes, el, bdy, ip, ix, M, K, f, LM, coefK = tube_assembly(10, 6, S=3)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((3 * nodes.shape[0], 3)) * 1e-4
+    up = u + rng.standard_normal(u.shape) * 1e-6
+    bd = (3 * bdy[:, None] + np.arange(3)).ravel()
+    p = 1e4 * (1.0 + 0.3 * np.sin(np.arange(12)))
+    for beta, damp in ((0.0, 0.0), (1e-5, 3.0)):
+        a, b, hist = u.copy(), up.copy(), []
+        for k in range(p.size):
+            a, b = co.step(ip, ix, K, LM, LM, f, p[k], a, b, 1e-7, beta=beta, damp=damp,
+                           bdy_dofs=bd, threads=2), a
+            hist.append(a)
+        ur, upr, snaps = co.run(ip, ix, K, LM, LM, f, p, u, up, 1e-7, beta=beta, damp=damp,
+                                bdy_dofs=bd, snap_steps=(1, 7, 12), threads=3)
+        assert np.array_equal(ur, a) and np.array_equal(upr, b)
+        for i, k in enumerate((1, 7, 12)):
+            assert np.array_equal(snaps[i], hist[k - 1])
+    with pytest.raises(ValueError):
+        co.run(ip, ix, K, LM, LM, f, p, u, up, 1e-7, snap_steps=(0,))
+
+
+def test_large_size_oracle_helper_equals_numpy_oracle():
+    """The helper the large GPU parity tests use (C assembly, closed-form mass and load, C long run
+    from rest) follows the NumPy oracle's assembly and run."""
+    from oracle_cases import oracle_trajectory
+    nodes, el, bdy = tube_geometry(12, 7, 0.4, 1.6)
+    N, S = nodes.shape[0], 3
+    rng = np.random.default_rng(8)
+    E = 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
+    ip, ix = so.sparse_info_loops(el, N)
+    _, gth = so.gauss_properties(np.full((N, S), 7e6), np.full((N, S), 0.07), el)
+    coefK = so.coef_k_element_E(gth, E)
+    _, K, f1, LM = so.assemble_reference(nodes, el, 1.0, so.d_matrix(0.5), coefK, gth, 1.0, ip, ix)
+    dt, beta = 1e-6, 2e-7
+    p = 1.33e4 * (1.0 + 0.1 * np.sin(np.arange(25) * 0.3))
+    bd = (3 * bdy[:, None] + np.arange(3)).ravel()
+    z = np.zeros(3 * N)
+    _, _, ref = so.run(ip, ix, K, LM, f1, p, z, z, dt, beta=beta, bdy_dofs=bd, record=(10, 25))
+    snaps = oracle_trajectory(nodes, el, bdy, E, p, dt, beta, 7e6, 0.07, 1.0, 0.5, ip, ix, (10, 25))
+    for i, k in enumerate((10, 25)):
+        assert np.max(np.abs(ref[k])) > 0
+        assert np.max(np.abs(snaps[i] - ref[k])) <= 1e-12 * np.max(np.abs(ref[k]))
+
+
 # ----------------------------------------------------------------------------------------------
 # Updated geometry and stress
 # ----------------------------------------------------------------------------------------------
