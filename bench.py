@@ -23,8 +23,12 @@ Other workloads (``--workload``):
            matrix-free steps timed, then 2,000 forward steps and the adjoint gradient with two-level
            checkpoints (every 200 steps, then every 20 inside a segment) and the observations in
            host memory, sized to fit 288 GB;
-  copy     a plain device copy of ``--copy-bytes`` (the calibration kernel for the counters), timed:
-           the streaming rate the box reaches.
+  copy     the hand-written streaming kernels (16-byte non-temporal and ordinary reads, a copy)
+           over ``--copy-bytes``: the streaming rate the box reaches, and the calibration kernels
+           for the traffic counters;
+  updated  the updated-geometry step (stiffness and follower load on X + u, re-formed every step
+           as the shipped CVFES cfgs do, and every 10 steps) of the 102,400-node tube, each kernel
+           form timed, with HBM and FP64 rooflines.
 """
 from __future__ import annotations
 
@@ -40,6 +44,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def reads_upper_blocks(sysm):
+    """Whether the stored step of ``sysm`` reads the symmetric (upper-block) storage: the system
+    holds only that storage, or its step kernel is one of the symmetric variants."""
+    from cvfes_amd import _lib
+    kern = int(sysm.lib.cvfes_step_kernel(sysm._sys, _lib.METHOD_STORED))
+    return sysm.values is None or kern in (_lib.VARIANT_SYM, _lib.VARIANT_SYM_NT, _lib.VARIANT_SYM_NTL)
 
 
 def algorithmic_bytes(sysm, method):
@@ -50,13 +63,29 @@ def algorithmic_bytes(sysm, method):
     small = 3 * N * 8 + 4 * (N + 1) + 4 * nnzb + N   # f1, indptr, indices, fixed
     if getattr(sysm, "row_order", None) is not None:
         small += 4 * N                                 # the visiting order
-    if method == "stored" and getattr(sysm, "variant", 0) in (8, 9, 10):  # symmetric storage
+    if method == "stored" and reads_upper_blocks(sysm):                  # symmetric storage
         op = sysm.nnzu * 9 * S * 8 + 4 * nnzb                          # upper values, vblk
     elif method == "stored":
         op = nnzb * 9 * S * 8
     else:
         op = E * S * 8 + 9 * E * 9 * 8 + 4 * 9 * E + 4 * (nnzb + 1)   # coef, khat, celem, cptr
     return state + lm + small + op
+
+
+def kernel_name(sysm, method):
+    """Name of the step kernel ``cvfes_step`` launches for ``sysm`` and ``method``."""
+    from cvfes_amd import _lib
+    kern = int(sysm.lib.cvfes_step_kernel(sysm._sys, _lib.METHODS[method]))
+    names = {v: k for k, v in _lib.VARIANTS.items()}
+    return f"{method} step, variant {names.get(kern, kern)}"
+
+
+def traffic_key(method, storage, variant, order):
+    """Key of a step kernel's counter summary in ``profiles/pmc_traffic.json``."""
+    key = method if method == "matfree" else f"stored-{storage}"
+    if variant != "auto":
+        key += f"-{variant}"
+    return key if order == "natural" else f"{key}-{order}"
 
 
 def measured_traffic(workload, kernel):
@@ -148,7 +177,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="plain",
-                    choices=["plain", "batch", "adjoint", "combined", "copy"])
+                    choices=["plain", "batch", "adjoint", "combined", "copy", "updated"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
@@ -156,12 +185,15 @@ def main():
                     help="batch: scale the sample counts so all stored values fit at once")
     ap.add_argument("--case", default="tube-256x400-s128")
     ap.add_argument("--method", default="stored", choices=["stored", "matfree"])
+    ap.add_argument("--storage", default="sym", choices=["sym", "full"],
+                    help="stored values: upper blocks only (sym) or every block (full)")
     ap.add_argument("--variant", default="auto", help="step kernel variant (cvfes_amd._lib.VARIANTS)")
     ap.add_argument("--waves-per-simd", default="auto", choices=["auto", "0", "3", "4", "5", "6", "7"],
                     help="occupancy cap of the timed step (auto: the fastest of none, 6, 5, 4)")
     ap.add_argument("--row-order", default="auto", choices=["auto", "natural", "strips"],
                     help="row visiting order of the timed step (auto: the fastest of natural and "
                          "strips 32/64 on 20 steps each; the result does not depend on it)")
+    ap.add_argument("--strip-width", type=int, default=32, help="strip width of --row-order strips")
     ap.add_argument("--cpu-steps", type=int, default=22,
                     help="CPU baseline steps (the first two are warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
@@ -183,6 +215,8 @@ def main():
         return run_copy(args)
     if args.workload == "combined":
         return run_combined(args)
+    if args.workload == "updated":
+        return run_updated(args)
     args.steps = 200 if args.steps is None else args.steps
     args.warmup = 20 if args.warmup is None else args.warmup
 
@@ -199,7 +233,7 @@ def main():
     gm = sample_element_E(el, case.n_smp, seed=case.seed, E0=case.E0, cov=case.cov, device="cuda")
     N, S = nodes.shape[0], case.n_smp
     sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=gm.E,
-                          density=case.density, nu=case.nu, store_values=True)
+                          density=case.density, nu=case.nu, store_values=True, storage=args.storage)
     sysm.set_variant(args.variant)
     total = args.warmup + args.steps
     load = sysm.loads(case.pressures(0, max(total, args.cpu_steps + 1), dt))
@@ -211,8 +245,8 @@ def main():
     if args.row_order == "auto":
         order = sysm.autotune_row_order(load, dt, beta, method=args.method)
     else:
-        sysm.set_row_order(args.row_order, width=32)
-        order = {"order": args.row_order, "width": 32 if args.row_order == "strips" else 0}
+        sysm.set_row_order(args.row_order, width=args.strip_width)
+        order = {"order": args.row_order, "width": args.strip_width if args.row_order == "strips" else 0}
     order_name = order["order"] + (str(order["width"]) if order["order"] == "strips" else "")
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
@@ -236,14 +270,16 @@ def main():
                    "nnzb": sysm.nnzb, "n_samples": S, "dt": dt, "beta": beta,
                    "operator": args.method, "variant": args.variant, "row_order": order_name,
                    "waves_per_simd": occ["waves_per_simd"],
-                   "layout": "values [nnzb][9][S], state [N][3][S]"},
+                   "storage": args.storage if args.method == "stored" else None,
+                   "step_kernel": kernel_name(sysm, args.method),
+                   "layout": ("upper blocks uvalues [nnzu][9][S]" if args.storage == "sym"
+                              else "values [nnzb][9][S]") + ", state [N][3][S]"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(case.name, (args.method if args.variant == "auto"
-                                                             else f"{args.method}-{args.variant}")
-                                                 + ("" if order_name == "natural" else f"-{order_name}")),
+                     "traffic": measured_traffic(case.name, traffic_key(args.method, args.storage,
+                                                                        args.variant, order_name)),
                      "algorithmic_bytes_per_launch": abytes, "kernel_ms": kern_ms,
-                     "kernel": "k_step (fused block-CSR product + central-difference update)"},
+                     "kernel": kernel_name(sysm, args.method)},
     }
     # achievable streaming rate of this box with the step kernels' load forms (calibration kernels)
     ceil = stream_ceiling()
@@ -253,32 +289,42 @@ def main():
           f"{ceil['read_GBs']:.0f} GB/s reads, {ceil['copy_nt_GBs']:.0f} GB/s copy",
           file=sys.stderr, flush=True)
     if not args.no_matfree:
-        # the other operator and the symmetric storage of the stored one, measured the same way
-        alts = [("matfree", "auto", "natural")] if args.method == "stored" else [("stored", "auto", "natural")]
-        if args.method == "stored" and args.variant == "auto":
-            alts += [("stored", "sym_ntl", "natural"), ("stored", "sym_ntl", "strips"),
-                     ("matfree", "ebe", "natural")]
+        # the other operator, the other storage and order of the stored one, the element-by-element
+        # form, each measured the same way: (operator, storage, variant, row order)
+        if args.method == "stored":
+            other_storage = "full" if args.storage == "sym" else "sym"
+            alts = [("matfree", None, "auto", "natural"), ("stored", other_storage, "auto", "natural")]
+            if order_name != "natural":
+                alts.append(("stored", args.storage, args.variant, "natural"))
+            alts.append(("matfree", None, "ebe", "natural"))
+        else:
+            alts = [("stored", args.storage, "auto", "natural")]
         out["alternatives"] = []
         main_occ = sysm.waves_per_simd
-        for other, variant, order in alts:
+        for other, storage, variant, aorder in alts:
+            if storage is not None and storage != sysm.storage:
+                sysm.set_storage(storage)
             sysm.set_variant(variant)
-            sysm.set_row_order(order)
+            sysm.set_row_order(aorder)
             aocc = sysm.autotune_occupancy(load, dt, beta, method=other, steps=5)
             msteps = max(20, args.steps // 2)
             mwall, mk = time_steps(sysm, load, dt, beta, other, args.warmup, msteps)
             mbytes = algorithmic_bytes(sysm, other)
-            key = other if variant == "auto" else f"{other}-{variant}"
-            sysm.set_row_order("natural")
+            key = traffic_key(other, storage, variant, aorder)
             out["alternatives"].append({
-                "operator": other, "variant": variant, "row_order": order,
+                "operator": other, "storage": storage, "variant": variant, "row_order": aorder,
+                "kernel": kernel_name(sysm, other),
                 "waves_per_simd": aocc["waves_per_simd"], "value": S * msteps / mwall,
                 "unit": "sample-steps/s", "kernel_ms": mk, "achieved_GBs": mbytes / (mk * 1e-3) / 1e9,
                 "frac": mbytes / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": mbytes,
                 "traffic": measured_traffic(case.name, key)})
-            print(f"[bench] {key} ({order} order): {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
+            print(f"[bench] {key}: {mk:.4f} ms/kernel, {S * msteps / mwall:.0f} sample-steps/s",
                   file=sys.stderr, flush=True)
+        if sysm.storage != args.storage:
+            sysm.set_storage(args.storage)
         sysm.set_variant(args.variant)
         sysm.set_occupancy(main_occ)
+        sysm.set_row_order(order["order"], width=order["width"] or 32)
     if not args.no_cpu:
         from oracle import cpu_oracle as co
         threads = args.cpu_threads or co.max_threads()
@@ -454,6 +500,92 @@ def run_batch(args):
         "speedup_vs_per_mesh_matfree": t_mf / t_batch,
         "parity_vs_per_mesh_matfree": {"steps": 2 * block, "rel_max_err": worst},
     }
+    print(json.dumps(out), flush=True)
+
+
+# FP64 operations of the updated-geometry step per (row node, sample, incident element), FMA = 2,
+# square root and division = 1 each: frame (edge vectors 6, two normalisations 2 x (5 + 1 + 1 + 3),
+# projection 5 + 6, cross product 9), local coordinates 30, area 12, edge terms 6, element scale 4,
+# three blocks applied in the local frame 3 x 46, accumulation 21, follower load 9.
+UPDATED_FLOPS_PER_ROW_ELEMENT = 272
+UPDATED_FLOPS_PER_ROW = 20          # the central-difference update of the row (3 components)
+
+
+def updated_algorithmic_bytes(sysm, interval):
+    """HBM bytes one updated-geometry step must move at minimum: coefficients, the three state
+    arrays, the geometry displacement when it is a saved copy (interval > 1), the reference
+    coordinates, connectivity, node-element lists, lumped mass and boundary flags (the element
+    blocks and the load are formed from the geometry, not read)."""
+    N, S, E = sysm.n_nodes, sysm.n_smp, sysm.n_elms
+    b = E * S * 8 + 3 * (3 * N * S * 8)                     # coef; read u, up, write u+
+    if interval > 1:
+        b += 3 * N * S * 8                                 # ug
+    b += 24 * N + 12 * E + 4 * (N + 1) + 12 * E            # nodes, elems, nptr, nent
+    b += sysm.lm.numel() * 8 + N                           # lumped mass, fixed
+    return b
+
+
+def run_updated(args):
+    """Updated-geometry step (stiffness and follower load on X + u) of the 102,400-node tube with
+    128 samples: every kernel form at re-formation interval 1 (every step, what the shipped CVFES
+    cfgs select) and the fastest at interval 10; HBM and FP64 rooflines."""
+    import torch
+    from cvfes_amd.configs import get_case
+    from cvfes_amd.gmrf import sample_element_E
+    from cvfes_amd.system import MembraneSystem
+
+    case = get_case(args.case)
+    nodes, el, bdy = case.geometry()
+    N, S, E = nodes.shape[0], case.n_smp, el.shape[0]
+    gm = sample_element_E(el, S, seed=case.seed, E0=case.E0, cov=case.cov, device="cuda")
+    sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=gm.E,
+                          density=case.density, nu=case.nu, store_values=False)
+    dt, beta = case.dt, case.beta
+    steps = 100 if args.steps is None else args.steps
+    warm = 5 if args.warmup is None else args.warmup
+    load = sysm.loads(case.pressures(0, 2 * (warm + steps) + 10, dt))
+    flops = (UPDATED_FLOPS_PER_ROW_ELEMENT * 3 * E + UPDATED_FLOPS_PER_ROW * N) * S
+    st = torch.cuda.current_stream()
+    res = {}
+    for interval in (1, 10):
+        forms = ("generic", "rowload1", "rowload") if interval == 1 else (best_form,)
+        for form in forms:
+            sysm.set_variant(form)
+            z = np.zeros(3 * N)
+            sysm.initial_state(z, z, load, dt, beta=beta, method="matfree")
+            sysm.step_updated(load, dt, interval, beta=beta, offset=0, nsteps=warm + 1)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(st)
+            sysm.step_updated(load, dt, interval, beta=beta, offset=warm + 1, nsteps=steps)
+            e1.record(st)
+            torch.cuda.synchronize()
+            if not bool(torch.isfinite(sysm.u).all()):
+                raise RuntimeError(f"updated step ({form}): non-finite displacement")
+            ms = e0.elapsed_time(e1) / steps
+            abytes = updated_algorithmic_bytes(sysm, interval)
+            res[f"interval{interval}-{form}"] = {
+                "kernel_ms": ms, "sample_steps_per_s": S / (ms * 1e-3),
+                "hbm": {"achieved_GBs": abytes / (ms * 1e-3) / 1e9, "frac": abytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_launch": abytes},
+                "fp64": {"achieved_TFLOPs": flops / (ms * 1e-3) / 1e12,
+                         "frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops_per_launch": flops}}
+            print(f"[bench] updated interval {interval} {form}: {ms:.4f} ms/step, "
+                  f"{flops / (ms * 1e-3) / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        if interval == 1:
+            best_form = min(("generic", "rowload1", "rowload"), key=lambda f: res[f"interval1-{f}"]["kernel_ms"])
+    sysm.set_variant("auto")
+    best = res[f"interval1-{best_form}"]
+    out = {"metric": "updated-geometry ensemble sample-steps/s at 102k nodes x 128 samples",
+           "value": best["sample_steps_per_s"], "unit": "sample-steps/s", "n_gpus": 1, "steps": steps,
+           "warmup": warm, "ms_per_step": best["kernel_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded tube mesh and GMRF samples",
+           "config": {"workload": case.name + "-updated", "interval": 1, "kernel_form": best_form,
+                      "beta": beta, "dt": dt},
+           "roofline": {"bound": "fp64", "achieved": best["fp64"]["achieved_TFLOPs"],
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": best["fp64"]["frac"],
+                        "hbm_frac": best["hbm"]["frac"], "flops_per_launch": flops},
+           "forms": res}
     print(json.dumps(out), flush=True)
 
 

@@ -173,7 +173,7 @@ class BatchedEnsemble:
                  occupancies=(0, 6, 5, 4)):
         """Pick the lane width, register cap and grid size (wavefronts) with the shortest step,
         then the cap on the waves per SIMD for that choice."""
-        n_grid = len(candidates) if candidates is not None else 3
+        n_grid = len(candidates) if candidates is not None else 5
         n_occ = sum(1 for o in occupancies if o != 0)
         # every timed candidate runs 1 + steps eager steps of the ensemble's load tables
         self._check_steps((len(lane_widths) * len(caps) * n_grid + n_occ) * (steps + 1))
@@ -184,7 +184,11 @@ class BatchedEnsemble:
             self.vw_hint = vw
             self._sync_roles()
             self._replan()
-            grid = candidates if candidates is not None else [cus * 16, cus * 64, self.n_tasks]
+            # persistent grids of 16 and 64 wavefronts per CU, a wavefront per 16 or 4 tasks (the
+            # descriptor lookup amortised over consecutive rows of one mesh), one per task
+            grid = candidates if candidates is not None else [
+                cus * 16, cus * 64, max(cus * 64, self.n_tasks // 16), max(cus * 64, self.n_tasks // 4),
+                self.n_tasks]
             for cap in caps:
                 self.cap = cap
                 for w in grid:

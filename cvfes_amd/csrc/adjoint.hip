@@ -101,7 +101,7 @@ __device__ __forceinline__ void adj_finish(const AdjArgs &a, int32_t row, int64_
     }
 }
 
-template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
+template <int OP, bool UNIFORM, bool BETA, bool DAMP, int VW>
 __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
     int32_t row;
     int64_t s;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
         const int64_t col = a.k.indices[p];
         double x[3][VW];
         adj_column<BETA, VW>(a, col, s, x);
-        block_apply<MATFREE, VW>(a.k, p, s, x, acc);
+        block_apply<OP, VW>(a.k, p, col < row, s, x, acc);
     }
     adj_finish<DAMP, VW>(a, row, s, active, acc);
 }
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs a) {
 // checkpoint segment runs with the recomputation of the segment before it).  (A register target
 // of 5 waves per SIMD, 94 VGPRs instead of 100 for the stored two-sample instance, measured 2-3 %
 // slower: the loads in flight per wave count for more here.)
-template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
+template <int OP, bool UNIFORM, bool BETA, bool DAMP, int VW>
 __global__ __launch_bounds__(kBlock) void k_fused_step(StepArgs f, AdjArgs a, int64_t step) {
     int32_t row;
     int64_t s;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(StepArgs f, AdjArgs a, in
         double xf[3][VW], xa[3][VW];
         state_column<BETA, VW>(f.u, f.up, f.bdt, col, f.k.n_smp, s, xf);
         adj_column<BETA, VW>(a, col, s, xa);
-        block_apply2<MATFREE, VW>(f.k, p, s, xf, accf, xa, acca);
+        block_apply2<OP, VW>(f.k, p, col < row, s, xf, accf, xa, acca);
     }
     step_update<DAMP, VW>(f, row, s, active, step, accf);
     adj_finish<DAMP, VW>(a, row, s, active, acca);
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_accum_lds(GradArgs a) {
     a.grad[g] -= a.coef[g] * acc;
 }
 
-template <bool MF, bool UNI, int VW>
+template <int MF, bool UNI, int VW>
 void adj_launch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
     const dim3 g(grid_for(a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp))), b(kBlock);
     if (beta) {
@@ -259,7 +259,7 @@ void adj_launch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
 
 // Lane width and row mapping as the generic forward kernel chooses them (two samples per lane
 // for even counts; wave-uniform rows when they keep 3/4 of the lanes busy).
-template <bool MF>
+template <int MF>
 void adj_dispatch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
     if (a.k.n_smp % 2 == 0) {
         if (use_uniform<2>(a.k.n_smp)) adj_launch<MF, true, 2>(st, a, beta, damp);
@@ -270,7 +270,7 @@ void adj_dispatch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
     }
 }
 
-template <bool MF, bool UNI, int VW>
+template <int MF, bool UNI, int VW>
 void fused_launch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta, bool damp) {
     const dim3 g(grid_for(f.k.n_nodes * lanes_per_row<UNI, VW>(f.k.n_smp))), b(kBlock);
     if (beta) {
@@ -282,7 +282,7 @@ void fused_launch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta
     }
 }
 
-template <bool MF>
+template <int MF>
 void fused_dispatch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta, bool damp) {
     if (f.k.n_smp % 2 == 0) {
         if (use_uniform<2>(f.k.n_smp)) fused_launch<MF, true, 2>(st, f, a, beta, damp);
@@ -293,13 +293,20 @@ void fused_dispatch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool be
     }
 }
 
+// Operator form of the generic row kernels for this system and method (as the forward step's
+// generic kernel chooses it: full values when present, else the symmetric upper blocks).
+int operator_form(const cvfes_system_t *sys, int method) {
+    return method == CVFES_METHOD_MATFREE ? kOpMatfree : (sys->values ? kOpStored : kOpSym);
+}
+
 int check_adjoint_args(const cvfes_system_t *sys, int method, const double *mu1, const double *mu2,
                        const double *mu_out, double dt, const char *who) {
     CVFES_REQUIRE(sys && sys->indptr && sys->indices && sys->n_nodes >= 1 && sys->n_smp >= 1,
                   "%s: empty system", who);
     CVFES_REQUIRE(method == CVFES_METHOD_STORED || method == CVFES_METHOD_MATFREE,
                   "%s: unknown method %d", who, method);
-    CVFES_REQUIRE(method == CVFES_METHOD_MATFREE || sys->values, "%s: no values", who);
+    CVFES_REQUIRE(method == CVFES_METHOD_MATFREE || sys->values || (sys->uvalues && sys->vblk),
+                  "%s: no stored values (full or symmetric)", who);
     CVFES_REQUIRE(method == CVFES_METHOD_STORED || (sys->cptr && sys->celem && sys->khat && sys->coef),
                   "%s: matrix-free needs cptr/celem/khat/coef", who);
     CVFES_REQUIRE(mu1 && mu2 && mu_out && sys->lm, "%s: null pointer", who);
@@ -315,7 +322,7 @@ AdjArgs adjoint_args(const cvfes_system_t *sys, const double *mu1, const double 
                      double beta, double damp, int project) {
     AdjArgs a{};
     a.k = RowArgs{sys->n_nodes, sys->n_smp, sys->indptr, sys->indices, sys->values,
-                  sys->cptr, sys->celem, sys->khat, sys->coef};
+                  sys->cptr, sys->celem, sys->khat, sys->coef, sys->uvalues, sys->vblk};
     a.mu1 = mu1;
     a.mu2 = mu2;
     a.out = mu_out;
@@ -346,10 +353,11 @@ extern "C" int cvfes_adjoint_step(void *stream, const cvfes_system_t *sys, int m
     if (int rc = check_adjoint_args(sys, method, mu1, mu2, mu_out, dt, "cvfes_adjoint_step")) return rc;
     const AdjArgs a = adjoint_args(sys, mu1, mu2, mu_out, obs_u, obs_ref, obs_w, dt, beta, damp, project);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (method == CVFES_METHOD_MATFREE)
-        adj_dispatch<true>(st, a, beta != 0.0, damp != 0.0);
-    else
-        adj_dispatch<false>(st, a, beta != 0.0, damp != 0.0);
+    switch (operator_form(sys, method)) {
+    case kOpMatfree: adj_dispatch<kOpMatfree>(st, a, beta != 0.0, damp != 0.0); break;
+    case kOpStored: adj_dispatch<kOpStored>(st, a, beta != 0.0, damp != 0.0); break;
+    default: adj_dispatch<kOpSym>(st, a, beta != 0.0, damp != 0.0); break;
+    }
     return check_launch("k_adjoint_step");
 }
 
@@ -376,7 +384,7 @@ extern "C" int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys,
                   static_cast<long long>(kern));
     StepArgs f{};
     f.k = RowArgs{sys->n_nodes, sys->n_smp, sys->indptr, sys->indices, sys->values,
-                  sys->cptr, sys->celem, sys->khat, sys->coef};
+                  sys->cptr, sys->celem, sys->khat, sys->coef, sys->uvalues, sys->vblk};
     f.u = state[0];
     f.up = state[1];
     f.out = state[2];
@@ -394,10 +402,11 @@ extern "C" int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys,
     f.nt_out = stream_out(sys->n_nodes, sys->n_smp);
     const AdjArgs a = adjoint_args(sys, mu1, mu2, mu_out, obs_u, obs_ref, obs_w, dt, beta, damp, 1);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (method == CVFES_METHOD_MATFREE)
-        fused_dispatch<true>(st, f, a, beta != 0.0, damp != 0.0);
-    else
-        fused_dispatch<false>(st, f, a, beta != 0.0, damp != 0.0);
+    switch (operator_form(sys, method)) {
+    case kOpMatfree: fused_dispatch<kOpMatfree>(st, f, a, beta != 0.0, damp != 0.0); break;
+    case kOpStored: fused_dispatch<kOpStored>(st, f, a, beta != 0.0, damp != 0.0); break;
+    default: fused_dispatch<kOpSym>(st, f, a, beta != 0.0, damp != 0.0); break;
+    }
     return check_launch("k_fused_step");
 }
 

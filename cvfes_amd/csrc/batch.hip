@@ -38,13 +38,14 @@ __device__ __forceinline__ const CVFES_K T *as_const(const T *p) {
     return (const CVFES_K T *)p;
 }
 
-template <bool MF, bool BETA, bool DAMP, int VW>
+template <int OP, bool BETA, bool DAMP, int VW>
 __device__ __forceinline__ void batch_task(const cvfes_batch_mesh_t &d, int phase, int64_t n,
                                            int32_t row, int64_t s, bool active) {
     GStepArgs a;
     a.k = GRowArgs{d.sys.n_nodes, d.sys.n_smp, as_const(d.sys.indptr), as_const(d.sys.indices),
                    as_global(d.sys.values), as_const(d.sys.cptr), as_const(d.sys.celem),
-                   as_const(d.sys.khat), as_global(d.sys.coef)};
+                   as_const(d.sys.khat), as_global(d.sys.coef), as_global(d.sys.uvalues),
+                   as_const(d.sys.vblk)};
     a.u = as_global(d.state[phase]);
     a.up = as_global(d.state[(phase + 2) % 3]);
     a.out = (CVFES_G double *)d.state[(phase + 1) % 3];
@@ -59,7 +60,7 @@ __device__ __forceinline__ void batch_task(const cvfes_batch_mesh_t &d, int phas
     a.bdt = d.beta / d.dt;
     a.ddt = d.damp / d.dt;
     a.nt_out = false;   // ordinary stores: the ensemble measured faster without streaming them
-    step_row<MF, BETA, DAMP, VW>(a, row, s, active, n);
+    step_row<OP, BETA, DAMP, VW>(a, row, s, active, n);
 }
 
 // Path sets: a kernel instance only contains the operators and damping forms that occur in the
@@ -71,19 +72,30 @@ __device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int 
     const bool beta = BD && d.beta != 0.0, damp = BD && d.damp != 0.0;
     if (mf) {
         if (beta) {
-            if (damp) batch_task<true, true, true, VW>(d, phase, n, row, s, active);
-            else batch_task<true, true, false, VW>(d, phase, n, row, s, active);
+            if (damp) batch_task<kOpMatfree, true, true, VW>(d, phase, n, row, s, active);
+            else batch_task<kOpMatfree, true, false, VW>(d, phase, n, row, s, active);
         } else {
-            if (damp) batch_task<true, false, true, VW>(d, phase, n, row, s, active);
-            else batch_task<true, false, false, VW>(d, phase, n, row, s, active);
+            if (damp) batch_task<kOpMatfree, false, true, VW>(d, phase, n, row, s, active);
+            else batch_task<kOpMatfree, false, false, VW>(d, phase, n, row, s, active);
         }
     } else if constexpr (STORED) {
-        if (beta) {
-            if (damp) batch_task<false, true, true, VW>(d, phase, n, row, s, active);
-            else batch_task<false, true, false, VW>(d, phase, n, row, s, active);
+        // stored values in full storage when the mesh holds them, else in symmetric storage
+        if (d.sys.values) {
+            if (beta) {
+                if (damp) batch_task<kOpStored, true, true, VW>(d, phase, n, row, s, active);
+                else batch_task<kOpStored, true, false, VW>(d, phase, n, row, s, active);
+            } else {
+                if (damp) batch_task<kOpStored, false, true, VW>(d, phase, n, row, s, active);
+                else batch_task<kOpStored, false, false, VW>(d, phase, n, row, s, active);
+            }
         } else {
-            if (damp) batch_task<false, false, true, VW>(d, phase, n, row, s, active);
-            else batch_task<false, false, false, VW>(d, phase, n, row, s, active);
+            if (beta) {
+                if (damp) batch_task<kOpSym, true, true, VW>(d, phase, n, row, s, active);
+                else batch_task<kOpSym, true, false, VW>(d, phase, n, row, s, active);
+            } else {
+                if (damp) batch_task<kOpSym, false, true, VW>(d, phase, n, row, s, active);
+                else batch_task<kOpSym, false, false, VW>(d, phase, n, row, s, active);
+            }
         }
     }
 }
@@ -235,7 +247,8 @@ extern "C" int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_
         CVFES_REQUIRE(s.n_nodes * 3 < (int64_t(1) << 31), "cvfes_batch_plan: mesh %d too large", i);
         CVFES_REQUIRE(m.method == CVFES_METHOD_STORED || m.method == CVFES_METHOD_MATFREE,
                       "cvfes_batch_plan: mesh %d unknown method", i);
-        CVFES_REQUIRE(m.method == CVFES_METHOD_MATFREE || s.values, "cvfes_batch_plan: mesh %d has no values", i);
+        CVFES_REQUIRE(m.method == CVFES_METHOD_MATFREE || s.values || (s.uvalues && s.vblk),
+                      "cvfes_batch_plan: mesh %d has no stored values", i);
         CVFES_REQUIRE(m.method == CVFES_METHOD_STORED || (s.cptr && s.celem && s.khat && s.coef),
                       "cvfes_batch_plan: mesh %d lacks matrix-free data", i);
         CVFES_REQUIRE(m.state[0] && m.state[1] && m.state[2], "cvfes_batch_plan: mesh %d state", i);

@@ -20,14 +20,14 @@
 namespace cvfes {
 namespace {
 
-template <bool MATFREE, bool UNIFORM, int VW>
+template <int OP, bool UNIFORM, int VW>
 __global__ __launch_bounds__(kBlock) void k_spmv(StepArgs a) {
     int32_t row;
     int64_t s;
     bool active;
     if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
     double acc[3][VW];
-    row_product<MATFREE, false, VW>(a.k, a.u, static_cast<const double *>(nullptr), 0.0, row, s, acc);
+    row_product<OP, false, VW>(a.k, a.u, static_cast<const double *>(nullptr), 0.0, row, s, acc);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         Vec<VW> o;
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(StepArgs a) {
     }
 }
 
-template <bool MATFREE, bool UNIFORM, bool BETA, bool DAMP, int VW>
+template <int OP, bool UNIFORM, bool BETA, bool DAMP, int VW>
 __global__ __launch_bounds__(kBlock) void k_step(StepArgs a, int64_t step) {
     int32_t row;
     int64_t s;
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a, int64_t step) {
     if constexpr (UNIFORM) {
         if (a.order) row = a.order[row];
     }
-    step_row<MATFREE, BETA, DAMP, VW>(a, row, s, active, step);
+    step_row<OP, BETA, DAMP, VW>(a, row, s, active, step);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -338,7 +338,8 @@ int validate(const cvfes_system_t *sys, int method, const char *who) {
     CVFES_REQUIRE(method == CVFES_METHOD_STORED || method == CVFES_METHOD_MATFREE,
                   "%s: unknown method %d", who, method);
     if (method == CVFES_METHOD_STORED)
-        CVFES_REQUIRE(sys->values, "%s: stored method needs values", who);
+        CVFES_REQUIRE(sys->values || (sys->uvalues && sys->vblk),
+                      "%s: stored method needs values (full, or symmetric: uvalues and vblk)", who);
     else
         CVFES_REQUIRE(sys->cptr && sys->celem && sys->khat && sys->coef,
                       "%s: matrix-free method needs cptr/celem/khat/coef", who);
@@ -349,59 +350,76 @@ int validate(const cvfes_system_t *sys, int method, const char *who) {
 
 RowArgs row_args(const cvfes_system_t *sys) {
     return RowArgs{sys->n_nodes, sys->n_smp, sys->indptr, sys->indices, sys->values,
-                   sys->cptr,    sys->celem, sys->khat,   sys->coef};
+                   sys->cptr,    sys->celem, sys->khat,   sys->coef, sys->uvalues, sys->vblk};
 }
 
 // Dispatch helpers ------------------------------------------------------------------------------
 // Wave-uniform rows unless padding to whole wavefronts would idle more than a quarter of the lanes.
 
-template <bool MF, bool UNI, int VW>
+template <int OP, bool UNI, int VW>
 void launch_spmv(hipStream_t st, const StepArgs &a) {
     const int64_t work = a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp);
-    hipLaunchKernelGGL((k_spmv<MF, UNI, VW>), dim3(grid_for(work)), dim3(kBlock), a.occ_lds, st, a);
+    hipLaunchKernelGGL((k_spmv<OP, UNI, VW>), dim3(grid_for(work)), dim3(kBlock), a.occ_lds, st, a);
 }
 
-template <bool MF, int VW>
+template <int OP, int VW>
 void launch_spmv_u(hipStream_t st, const StepArgs &a) {
     if (use_uniform<VW>(a.k.n_smp))
-        launch_spmv<MF, true, VW>(st, a);
+        launch_spmv<OP, true, VW>(st, a);
     else
-        launch_spmv<MF, false, VW>(st, a);
+        launch_spmv<OP, false, VW>(st, a);
+}
+
+template <int OP>
+void spmv_op(hipStream_t st, const StepArgs &a) {
+    if (a.k.n_smp % 2 == 0) launch_spmv_u<OP, 2>(st, a);
+    else launch_spmv_u<OP, 1>(st, a);
+}
+
+// Operator form of the generic row kernels: matrix-free, or the stored values in the storage
+// the system holds (full values when present, else the symmetric upper blocks).
+int row_op(const RowArgs &k, bool mf) {
+    return mf ? kOpMatfree : (k.values ? kOpStored : kOpSym);
 }
 
 void spmv_dispatch(hipStream_t st, bool mf, const StepArgs &a) {
-    const bool v2 = a.k.n_smp % 2 == 0;
-    if (mf) {
-        if (v2) launch_spmv_u<true, 2>(st, a); else launch_spmv_u<true, 1>(st, a);
-    } else {
-        if (v2) launch_spmv_u<false, 2>(st, a); else launch_spmv_u<false, 1>(st, a);
+    switch (row_op(a.k, mf)) {
+    case kOpMatfree: spmv_op<kOpMatfree>(st, a); break;
+    case kOpStored: spmv_op<kOpStored>(st, a); break;
+    default: spmv_op<kOpSym>(st, a); break;
     }
 }
 
-template <bool MF, bool UNI, bool BETA, bool DAMP, int VW>
+template <int OP, bool UNI, bool BETA, bool DAMP, int VW>
 void launch_step(hipStream_t st, const StepArgs &a, int64_t step) {
     const int64_t work = a.k.n_nodes * lanes_per_row<UNI, VW>(a.k.n_smp);
-    hipLaunchKernelGGL((k_step<MF, UNI, BETA, DAMP, VW>), dim3(grid_for(work)), dim3(kBlock), a.occ_lds,
+    hipLaunchKernelGGL((k_step<OP, UNI, BETA, DAMP, VW>), dim3(grid_for(work)), dim3(kBlock), a.occ_lds,
                        st, a, step);
 }
 
-template <bool MF, bool UNI, int VW>
+template <int OP, bool UNI, int VW>
 void step_bd(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
     if (beta) {
-        if (damp) launch_step<MF, UNI, true, true, VW>(st, a, step);
-        else launch_step<MF, UNI, true, false, VW>(st, a, step);
+        if (damp) launch_step<OP, UNI, true, true, VW>(st, a, step);
+        else launch_step<OP, UNI, true, false, VW>(st, a, step);
     } else {
-        if (damp) launch_step<MF, UNI, false, true, VW>(st, a, step);
-        else launch_step<MF, UNI, false, false, VW>(st, a, step);
+        if (damp) launch_step<OP, UNI, false, true, VW>(st, a, step);
+        else launch_step<OP, UNI, false, false, VW>(st, a, step);
     }
 }
 
-template <bool MF, int VW>
+template <int OP, int VW>
 void step_u(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
     if (use_uniform<VW>(a.k.n_smp))
-        step_bd<MF, true, VW>(st, a, step, beta, damp);
+        step_bd<OP, true, VW>(st, a, step, beta, damp);
     else
-        step_bd<MF, false, VW>(st, a, step, beta, damp);
+        step_bd<OP, false, VW>(st, a, step, beta, damp);
+}
+
+template <int OP>
+void step_generic(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
+    if (a.k.n_smp % 2 == 0) step_u<OP, 2>(st, a, step, beta, damp);
+    else step_u<OP, 1>(st, a, step, beta, damp);
 }
 
 template <bool MF, bool NT, int MAXB, int VW>
@@ -627,11 +645,19 @@ int64_t resolve_variant(const cvfes_system_t *sys, bool mf) {
     const int maxb = rowload_slots(sys);
     const int64_t S = sys->n_smp;
     int64_t v = sys->variant;
-    const int64_t fallback = mf ? CVFES_VARIANT_GENERIC : CVFES_VARIANT_ROWLOAD_NT;
+    // Stored values in symmetric storage only (no full copy): the symmetric kernel, or the generic
+    // one, which reads either storage.
+    const bool sym_only = !mf && !sys->values && sys->uvalues && sys->vblk;
     // Measured on MI355X (102,400-node tube, 128 samples): the stored operator is fastest with the
+    // symmetric kernel whose mirrored reads are non-temporal, or, in full storage, with the
     // row-load kernel and non-temporal value loads; the matrix-free operator with the generic
     // kernel (the row-load form needs too many registers for enough waves in flight).
+    const int64_t fallback = mf ? CVFES_VARIANT_GENERIC
+                                : (sym_only ? (maxb ? CVFES_VARIANT_SYM_NTL : CVFES_VARIANT_GENERIC)
+                                            : CVFES_VARIANT_ROWLOAD_NT);
     if (v == CVFES_VARIANT_AUTO) v = fallback;
+    if (sym_only && v != CVFES_VARIANT_SYM && v != CVFES_VARIANT_SYM_NT && v != CVFES_VARIANT_SYM_NTL)
+        v = v == CVFES_VARIANT_GENERIC ? v : fallback;   // the full-storage kernels need `values`
     if (v == CVFES_VARIANT_LDS || v == CVFES_VARIANT_LDS2) {
         // LDS staging only exists for the matrix-free operator (and structures within its limits)
         if (mf && lds_slots(sys))
@@ -719,11 +745,10 @@ void step_dispatch(hipStream_t st, const cvfes_system_t *sys, bool mf, const Ste
     default:
         break;
     }
-    const bool v2 = a.k.n_smp % 2 == 0;
-    if (mf) {
-        if (v2) step_u<true, 2>(st, a, step, beta, damp); else step_u<true, 1>(st, a, step, beta, damp);
-    } else {
-        if (v2) step_u<false, 2>(st, a, step, beta, damp); else step_u<false, 1>(st, a, step, beta, damp);
+    switch (row_op(a.k, mf)) {
+    case kOpMatfree: step_generic<kOpMatfree>(st, a, step, beta, damp); break;
+    case kOpStored: step_generic<kOpStored>(st, a, step, beta, damp); break;
+    default: step_generic<kOpSym>(st, a, step, beta, damp); break;
     }
 }
 

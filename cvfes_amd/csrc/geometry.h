@@ -15,6 +15,9 @@ struct Frame {
     double area;
 };
 
+// RECIP: normalise with one reciprocal per vector (x * (1/|v|)) instead of three divisions; the
+// components then differ from x / |v| by at most an ulp.
+template <bool RECIP = false>
 __device__ __forceinline__ Frame frame_from_points(const double (&X)[3][3]) {
     Frame f;
     double e0[3], e1[3];
@@ -24,14 +27,26 @@ __device__ __forceinline__ Frame frame_from_points(const double (&X)[3][3]) {
         e1[k] = X[0][k] - X[2][k];
     }
     double nrm = sqrt(e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2]);
+    if constexpr (RECIP) {
+        const double inv = 1.0 / nrm;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) f.t[0][k] = e0[k] / nrm;
+        for (int k = 0; k < 3; ++k) f.t[0][k] = e0[k] * inv;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) f.t[0][k] = e0[k] / nrm;
+    }
     const double proj = e1[0] * f.t[0][0] + e1[1] * f.t[0][1] + e1[2] * f.t[0][2];
 #pragma unroll
     for (int k = 0; k < 3; ++k) f.t[1][k] = e1[k] - proj * f.t[0][k];
     nrm = sqrt(f.t[1][0] * f.t[1][0] + f.t[1][1] * f.t[1][1] + f.t[1][2] * f.t[1][2]);
+    if constexpr (RECIP) {
+        const double inv = 1.0 / nrm;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) f.t[1][k] = f.t[1][k] / nrm;
+        for (int k = 0; k < 3; ++k) f.t[1][k] = f.t[1][k] * inv;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) f.t[1][k] = f.t[1][k] / nrm;
+    }
     f.t[2][0] = f.t[0][1] * f.t[1][2] - f.t[0][2] * f.t[1][1];
     f.t[2][1] = f.t[0][2] * f.t[1][0] - f.t[0][0] * f.t[1][2];
     f.t[2][2] = f.t[0][0] * f.t[1][1] - f.t[0][1] * f.t[1][0];

@@ -38,6 +38,15 @@ inline bool use_uniform(int64_t n_smp) {
     return 4 * l >= 3 * padded;
 }
 
+// Operator forms of the row functions (template argument OP): stored values in full storage,
+// matrix-free (element coefficients x unit element blocks), stored values in symmetric storage.
+constexpr int kOpStored = 0;
+constexpr int kOpMatfree = 1;
+constexpr int kOpSym = 2;
+
+// Stored operator: full storage `values` [nnzb][9][S], or, when `values` is null, symmetric
+// storage: the upper blocks `uvalues` [nnzu][9][S], block p of the row structure being
+// uvalues[vblk[p]], read transposed when its column precedes its row.
 struct RowArgs {
     int64_t n_nodes;
     int64_t n_smp;
@@ -48,6 +57,8 @@ struct RowArgs {
     const int32_t *celem;
     const double *khat;
     const double *coef;
+    const double *uvalues;
+    const int32_t *vblk;
 };
 
 struct StepArgs {
@@ -83,6 +94,8 @@ struct GRowArgs {
     const CVFES_K int32_t *celem;
     const CVFES_K double *khat;
     const CVFES_G double *coef;
+    const CVFES_G double *uvalues;
+    const CVFES_K int32_t *vblk;
 };
 
 struct GLoad {
@@ -106,15 +119,28 @@ struct GStepArgs {
     bool nt_out;
 };
 
-// Stored block p of the row: kv[ij][v] = K_p[ij] of sample s + v.
-template <int VW, class RA>
-__device__ __forceinline__ void stored_block(const RA &k, int32_t p, int64_t s, double (&kv)[9][VW]) {
-    const auto *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
+// Stored block p of the row: kv[ij][v] = K_p[ij] of sample s + v.  SYM: symmetric storage, the
+// block is the upper block vblk[p], transposed when `tr` (its column precedes its row).
+template <bool SYM, int VW, class RA>
+__device__ __forceinline__ void stored_block(const RA &k, int32_t p, bool tr, int64_t s,
+                                             double (&kv)[9][VW]) {
+    if constexpr (!SYM) {
+        const auto *base = k.values + (static_cast<int64_t>(p) * 9) * k.n_smp + s;
 #pragma unroll
-    for (int ij = 0; ij < 9; ++ij) {
-        const Vec<VW> t = load_vec<VW>(base + ij * k.n_smp);
+        for (int ij = 0; ij < 9; ++ij) {
+            const Vec<VW> t = load_vec<VW>(base + ij * k.n_smp);
 #pragma unroll
-        for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
+            for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
+        }
+    } else {
+        const auto *base = k.uvalues + (static_cast<int64_t>(k.vblk[p]) * 9) * k.n_smp + s;
+#pragma unroll
+        for (int ij = 0; ij < 9; ++ij) {
+            const int src = tr ? 3 * (ij % 3) + ij / 3 : ij;
+            const Vec<VW> t = load_vec<VW>(base + src * k.n_smp);
+#pragma unroll
+            for (int v = 0; v < VW; ++v) kv[ij][v] = t.v[v];
+        }
     }
 }
 
@@ -122,12 +148,13 @@ __device__ __forceinline__ void stored_block(const RA &k, int32_t p, int64_t s, 
 // K_p = sum over the block's element contributions c of coef_e(c) * khat_c, applied as
 // acc += coef_e (khat_c x), so the block itself is never formed and only the three sums per
 // sample are live (khat_c is wave-uniform for wave-uniform rows: scalar loads).
-template <bool MATFREE, int VW, class RA>
-__device__ __forceinline__ void block_apply(const RA &k, int32_t p, int64_t s,
+// `tr`: the block's column precedes its row (symmetric storage reads it transposed).
+template <int OP, int VW, class RA>
+__device__ __forceinline__ void block_apply(const RA &k, int32_t p, bool tr, int64_t s,
                                             const double (&x)[3][VW], double (&acc)[3][VW]) {
-    if constexpr (!MATFREE) {
+    if constexpr (OP != kOpMatfree) {
         double kv[9][VW];
-        stored_block<VW>(k, p, s, kv);
+        stored_block<OP == kOpSym, VW>(k, p, tr, s, kv);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -174,13 +201,13 @@ __device__ __forceinline__ void state_column(DP u, DP up, double bdt, int64_t co
 
 // block_apply for two columns at once (the block is loaded once): acc += K_p x and acc2 += K_p x2,
 // each with exactly the operations block_apply performs.
-template <bool MATFREE, int VW, class RA>
-__device__ __forceinline__ void block_apply2(const RA &k, int32_t p, int64_t s,
+template <int OP, int VW, class RA>
+__device__ __forceinline__ void block_apply2(const RA &k, int32_t p, bool tr, int64_t s,
                                              const double (&x)[3][VW], double (&acc)[3][VW],
                                              const double (&x2)[3][VW], double (&acc2)[3][VW]) {
-    if constexpr (!MATFREE) {
+    if constexpr (OP != kOpMatfree) {
         double kv[9][VW];
-        stored_block<VW>(k, p, s, kv);
+        stored_block<OP == kOpSym, VW>(k, p, tr, s, kv);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -212,7 +239,7 @@ __device__ __forceinline__ void block_apply2(const RA &k, int32_t p, int64_t s,
 }
 
 // acc[i][v] = sum over the row's blocks of K_p (i, :) . x_col, with x = u + bdt (u - up) when BETA.
-template <bool MATFREE, bool BETA, int VW, class RA, class DP>
+template <int OP, bool BETA, int VW, class RA, class DP>
 __device__ __forceinline__ void row_product(const RA &k, DP u, DP up, double bdt, int32_t row,
                                             int64_t s, double (&acc)[3][VW]) {
 #pragma unroll
@@ -224,7 +251,7 @@ __device__ __forceinline__ void row_product(const RA &k, DP u, DP up, double bdt
         const int64_t col = k.indices[p];
         double x[3][VW];
         state_column<BETA, VW>(u, up, bdt, col, k.n_smp, s, x);
-        block_apply<MATFREE, VW>(k, p, s, x, acc);
+        block_apply<OP, VW>(k, p, col < row, s, x, acc);
     }
 }
 
@@ -269,11 +296,11 @@ __device__ __forceinline__ void step_update(const SA &a, int32_t row, int64_t s,
 }
 
 // Block-row product and central-difference update of one block row and VW consecutive samples.
-template <bool MATFREE, bool BETA, bool DAMP, int VW, class SA>
+template <int OP, bool BETA, bool DAMP, int VW, class SA>
 __device__ __forceinline__ void step_row(const SA &a, int32_t row, int64_t s, bool active,
                                          int64_t step) {
     double acc[3][VW];
-    row_product<MATFREE, BETA, VW>(a.k, a.u, a.up, a.bdt, row, s, acc);
+    row_product<OP, BETA, VW>(a.k, a.u, a.up, a.bdt, row, s, acc);
     step_update<DAMP, VW>(a, row, s, active, step, acc);
 }
 

@@ -40,68 +40,104 @@ __device__ __forceinline__ void moved_points(const GeomArgs &g, const int32_t (&
         }
 }
 
-template <bool UNIFORM, bool BETA, bool DAMP>
+// FAST: reciprocal-normalised frames and one reciprocal for the element scale (results within a few
+// ulp of the divided forms).  GEO_U: the geometry displacement is the current u (interval 1), so
+// each node's u is loaded once for the geometry and the product.  VW samples per lane.
+template <bool UNIFORM, bool BETA, bool DAMP, bool FAST, bool GEO_U, int VW>
 __global__ __launch_bounds__(kBlock) void k_step_updated(StepArgs a, GeomArgs g, int64_t step) {
     int32_t row;
     int64_t s;
     bool active;
-    if (!row_and_sample<UNIFORM, 1>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
+    if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
     const int64_t S = a.k.n_smp;
-    double acc[3] = {0.0, 0.0, 0.0};
-    double fl[3] = {0.0, 0.0, 0.0};      // unit-traction load of the row node on the moved geometry
+    double acc[VW][3] = {};
+    double fl[VW][3] = {};    // unit-traction load of the row node on the moved geometry
     for (int32_t q = g.nptr[row]; q < g.nptr[row + 1]; ++q) {
         const int32_t ent = g.nent[q];
         const int32_t e = ent / 3;
         const int la = ent - 3 * e;
         const int32_t ids[3] = {g.elems[3 * e], g.elems[3 * e + 1], g.elems[3 * e + 2]};
-        double X[3][3];
-        moved_points(g, ids, S, s, X);
-        const Frame f = frame_from_points(X);
-        const double bb[3] = {f.ly[1] - f.ly[2], f.ly[2] - f.ly[0], f.ly[0] - f.ly[1]};
-        const double cc[3] = {f.lx[2] - f.lx[1], f.lx[0] - f.lx[2], f.lx[1] - f.lx[0]};
-        const double scale = a.k.coef[static_cast<int64_t>(e) * S + s] * g.inv_1mnu2 / (4.0 * f.area);
-        double L0 = 0.0, L1 = 0.0, L2 = 0.0;
+        double ref[3][3];
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            double x[3];
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ref[b][k] = g.nodes[3 * static_cast<int64_t>(ids[b]) + k];
+        // per-sample state of the element's nodes: u (and up), the geometry displacement
+        double xu[VW][3][3], xg[VW][3][3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int64_t off = (static_cast<int64_t>(ids[b]) * 3 + k) * S + s;
-                const double xu = a.u[off];
-                x[k] = BETA ? xu + a.bdt * (xu - a.up[off]) : xu;
-            }
-            const double x0 = f.t[0][0] * x[0] + f.t[0][1] * x[1] + f.t[0][2] * x[2];
-            const double x1 = f.t[1][0] * x[0] + f.t[1][1] * x[1] + f.t[1][2] * x[2];
-            const double x2 = f.t[2][0] * x[0] + f.t[2][1] * x[1] + f.t[2][2] * x[2];
-            const double ba = bb[la], ca = cc[la];
-            L0 += (ba * bb[b] + g.g * ca * cc[b]) * x0 + (g.nu * ba * cc[b] + g.g * ca * bb[b]) * x1;
-            L1 += (g.nu * ca * bb[b] + g.g * ba * cc[b]) * x0 + (ca * cc[b] + g.g * ba * bb[b]) * x1;
-            L2 += g.gk * (ba * bb[b] + ca * cc[b]) * x2;
-        }
+                const Vec<VW> uu = load_vec<VW>(a.u + off);
+                Vec<VW> pp;
+                if constexpr (BETA) pp = load_vec<VW>(a.up + off);
+                Vec<VW> gg;
+                if constexpr (!GEO_U) {
+                    if (g.ug) gg = load_vec<VW>(g.ug + off);
+                }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            acc[k] += scale * (f.t[0][k] * L0 + f.t[1][k] * L1 + f.t[2][k] * L2);
-            fl[k] += (1.0 / 3.0) * f.area * f.t[2][k];
+                for (int v = 0; v < VW; ++v) {
+                    xu[v][b][k] = BETA ? uu.v[v] + a.bdt * (uu.v[v] - pp.v[v]) : uu.v[v];
+                    if constexpr (GEO_U) xg[v][b][k] = uu.v[v];
+                    else xg[v][b][k] = g.ug ? gg.v[v] : 0.0;
+                }
+            }
+        const Vec<VW> cf = load_vec<VW>(a.k.coef + static_cast<int64_t>(e) * S + s);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+            double X[3][3];
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) X[b][k] = ref[b][k] + xg[v][b][k];
+            const Frame f = frame_from_points<FAST>(X);
+            const double bb[3] = {f.ly[1] - f.ly[2], f.ly[2] - f.ly[0], f.ly[0] - f.ly[1]};
+            const double cc[3] = {f.lx[2] - f.lx[1], f.lx[0] - f.lx[2], f.lx[1] - f.lx[0]};
+            const double scale = FAST ? cf.v[v] * g.inv_1mnu2 * (1.0 / (4.0 * f.area))
+                                      : cf.v[v] * g.inv_1mnu2 / (4.0 * f.area);
+            double L0 = 0.0, L1 = 0.0, L2 = 0.0;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const double *x = xu[v][b];
+                const double x0 = f.t[0][0] * x[0] + f.t[0][1] * x[1] + f.t[0][2] * x[2];
+                const double x1 = f.t[1][0] * x[0] + f.t[1][1] * x[1] + f.t[1][2] * x[2];
+                const double x2 = f.t[2][0] * x[0] + f.t[2][1] * x[1] + f.t[2][2] * x[2];
+                const double ba = bb[la], ca = cc[la];
+                L0 += (ba * bb[b] + g.g * ca * cc[b]) * x0 + (g.nu * ba * cc[b] + g.g * ca * bb[b]) * x1;
+                L1 += (g.nu * ca * bb[b] + g.g * ba * cc[b]) * x0 + (ca * cc[b] + g.g * ba * bb[b]) * x1;
+                L2 += g.gk * (ba * bb[b] + ca * cc[b]) * x2;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                acc[v][k] += scale * (f.t[0][k] * L0 + f.t[1][k] * L1 + f.t[2][k] * L2);
+                fl[v][k] += (1.0 / 3.0) * f.area * f.t[2][k];
+            }
         }
     }
     const double pn = a.load.p[step];
     const double qa = a.load.ga ? a.load.qa[step] : 0.0;
     const double qb = a.load.gb ? a.load.qb[step] : 0.0;
     const bool fixed = a.fixed != nullptr && a.fixed[row] != 0;
-    const double m = a.lm_sstride ? a.lm[static_cast<int64_t>(row) * S + s] : a.lm[row];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int64_t off = (static_cast<int64_t>(row) * 3 + i) * S + s;
-        const double uu = a.u[off], pp = a.up[off];
+        const Vec<VW> uu = load_vec<VW>(a.u + off);
+        const Vec<VW> pp = load_vec<VW>(a.up + off);
         const int64_t d = 3 * static_cast<int64_t>(row) + i;
-        double force = pn * fl[i];
-        if (a.load.ga) force += qa * a.load.ga[d];
-        if (a.load.gb) force += qb * a.load.gb[d];
-        if constexpr (DAMP) force -= a.ddt * (uu - pp);
-        const double rhs = a.dt2 * (force - acc[i]) + m * (2.0 * uu - pp);
-        double un = m != 0.0 ? rhs / m : 0.0;
-        if (fixed) un = a.bdy_u;
-        if (active) a.out[off] = un;
+        Vec<VW> o;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+            const double m = a.lm_sstride ? a.lm[static_cast<int64_t>(row) * S + s + v] : a.lm[row];
+            double force = pn * fl[v][i];
+            if (a.load.ga) force += qa * a.load.ga[d];
+            if (a.load.gb) force += qb * a.load.gb[d];
+            if constexpr (DAMP) force -= a.ddt * (uu.v[v] - pp.v[v]);
+            const double rhs = a.dt2 * (force - acc[v][i]) + m * (2.0 * uu.v[v] - pp.v[v]);
+            const double un = m != 0.0 ? rhs / m : 0.0;
+            o.v[v] = fixed ? a.bdy_u : un;
+        }
+        if (active) store_vec<VW>(a.out + off, o);
     }
 }
 
@@ -172,17 +208,42 @@ GeomArgs geom_args(const cvfes_geometry_t *geo, const cvfes_system_t *sys, const
     return g;
 }
 
-template <bool UNIFORM>
-void launch_updated(hipStream_t st, const StepArgs &a, const GeomArgs &g, int64_t step, bool beta,
-                    bool damp) {
-    const int64_t work = a.k.n_nodes * lanes_per_row<UNIFORM, 1>(a.k.n_smp);
+template <bool UNIFORM, bool FAST, bool GEO_U, int VW>
+void launch_updated_inst(hipStream_t st, const StepArgs &a, const GeomArgs &g, int64_t step, bool beta,
+                         bool damp) {
+    const int64_t work = a.k.n_nodes * lanes_per_row<UNIFORM, VW>(a.k.n_smp);
     const dim3 grid(grid_for(work)), block(kBlock);
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_step_updated<UNIFORM, true, true>), grid, block, 0, st, a, g, step);
-        else hipLaunchKernelGGL((k_step_updated<UNIFORM, true, false>), grid, block, 0, st, a, g, step);
+        if (damp) hipLaunchKernelGGL((k_step_updated<UNIFORM, true, true, FAST, GEO_U, VW>), grid, block, 0, st, a, g, step);
+        else hipLaunchKernelGGL((k_step_updated<UNIFORM, true, false, FAST, GEO_U, VW>), grid, block, 0, st, a, g, step);
     } else {
-        if (damp) hipLaunchKernelGGL((k_step_updated<UNIFORM, false, true>), grid, block, 0, st, a, g, step);
-        else hipLaunchKernelGGL((k_step_updated<UNIFORM, false, false>), grid, block, 0, st, a, g, step);
+        if (damp) hipLaunchKernelGGL((k_step_updated<UNIFORM, false, true, FAST, GEO_U, VW>), grid, block, 0, st, a, g, step);
+        else hipLaunchKernelGGL((k_step_updated<UNIFORM, false, false, FAST, GEO_U, VW>), grid, block, 0, st, a, g, step);
+    }
+}
+
+template <bool FAST, bool GEO_U, int VW>
+void launch_updated_u(hipStream_t st, const StepArgs &a, const GeomArgs &g, int64_t step, bool beta,
+                      bool damp) {
+    if (use_uniform<VW>(a.k.n_smp)) launch_updated_inst<true, FAST, GEO_U, VW>(st, a, g, step, beta, damp);
+    else launch_updated_inst<false, FAST, GEO_U, VW>(st, a, g, step, beta, damp);
+}
+
+// Updated-geometry kernel forms (sys->variant): GENERIC the divided frames with one sample per
+// lane; ROWLOAD1 reciprocal frames, one sample per lane; ROWLOAD (and AUTO) reciprocal frames, two
+// samples per lane for even sample counts.  geo_u: the geometry displacement is the current u.
+void launch_updated(hipStream_t st, const StepArgs &a, const GeomArgs &g, int64_t step, bool beta,
+                    bool damp, int64_t variant, bool geo_u) {
+    const bool two = (variant == CVFES_VARIANT_AUTO || variant == CVFES_VARIANT_ROWLOAD) &&
+                     a.k.n_smp % 2 == 0;
+    if (variant == CVFES_VARIANT_GENERIC) {
+        launch_updated_u<false, false, 1>(st, a, g, step, beta, damp);
+    } else if (two) {
+        if (geo_u) launch_updated_u<true, true, 2>(st, a, g, step, beta, damp);
+        else launch_updated_u<true, false, 2>(st, a, g, step, beta, damp);
+    } else {
+        if (geo_u) launch_updated_u<true, true, 1>(st, a, g, step, beta, damp);
+        else launch_updated_u<true, false, 1>(st, a, g, step, beta, damp);
     }
 }
 
@@ -222,7 +283,6 @@ extern "C" int cvfes_step_updated(void *stream, const cvfes_system_t *sys, const
                   "cvfes_step_updated: load tables");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t S = sys->n_smp;
-    const bool uni = use_uniform<1>(S);
     double *cur = state[0], *prev = state[1], *nxt = state[2];
     for (int64_t n = 0; n < nsteps; ++n) {
         if (upd->counter >= upd->interval) {
@@ -240,7 +300,7 @@ extern "C" int cvfes_step_updated(void *stream, const cvfes_system_t *sys, const
         const double *geom = upd->has_geometry ? (upd->interval == 1 ? cur : upd->ug) : nullptr;
         StepArgs a;
         a.k = RowArgs{sys->n_nodes, S, sys->indptr, sys->indices, sys->values, sys->cptr, sys->celem,
-                      sys->khat, sys->coef};
+                      sys->khat, sys->coef, sys->uvalues, sys->vblk};
         a.u = cur;
         a.up = prev;
         a.out = nxt;
@@ -255,8 +315,7 @@ extern "C" int cvfes_step_updated(void *stream, const cvfes_system_t *sys, const
         a.ddt = damp / dt;
         a.order = nullptr;
         const GeomArgs g = geom_args(geo, sys, geom);
-        if (uni) launch_updated<true>(st, a, g, n, beta != 0.0, damp != 0.0);
-        else launch_updated<false>(st, a, g, n, beta != 0.0, damp != 0.0);
+        launch_updated(st, a, g, n, beta != 0.0, damp != 0.0, sys->variant, geom != nullptr && geom == cur);
         if (int rc = check_launch("k_step_updated")) return rc;
         double *t = prev;
         prev = cur;

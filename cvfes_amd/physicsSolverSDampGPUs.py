@@ -144,9 +144,14 @@ class GPUSolidSolver(PhysicsSolver):
         if self.updateInterval is None and self.kernelAutotune:
             # occupancy cap of the step kernel, timed on a constant load (the displacements do not
             # depend on it); the initial state below replaces the tuning steps' state
-            tune = self.system.loads(np.full(16, 1.0))
+            tune = self.system.loads(np.full(40, 1.0))
             self.tuned = self.system.autotune_occupancy(tune, self.dt, beta=self.beta, damp=self.damp,
                                                         method=self.method, steps=6)
+            if self.method == "stored":
+                # row visiting order: strips bring the mirrored reads of the symmetric storage
+                # within L2 reach on tubular walls (the result does not depend on the order)
+                self.tuned["row_order"] = self.system.autotune_row_order(
+                    tune, self.dt, beta=self.beta, damp=self.damp, method=self.method, steps=6)
         load0 = self._load_tables(np.array([self.time0]))
         # calc_ddu forms ddu_0 = (P_0 - K u_0) / LHS with no damping force (physicsSolverSDampGPUs.py
         # :232-246, .cl:398-409), so damp enters only the steps; beta (an extension, 0 unless set)

@@ -81,12 +81,18 @@ class MembraneSystem:
     coef : optional precomputed stiffness coefficient ``[E,S]`` (overrides vE/elmE).
     density, nu, kshear : material constants (``kshear`` = 5/6 as ``physicsSolver.py:177``).
     mass_form : ``"cpu"`` (oracle parity, default) or ``"gpu"`` lumping.
-    store_values : form the stored block-CSR values ``[nnzb][9][S]``.
+    store_values : form the stored block-CSR values.
+    storage : how the stored values are kept: ``"sym"`` (default) the upper blocks only,
+        ``uvalues [nnzu][9][S]`` (K is symmetric: a lower block is its mirror read transposed;
+        57 % of the full bytes), or ``"full"`` every block, ``values [nnzb][9][S]``.
     """
 
     def __init__(self, nodes, elements, boundary, n_smp, *, vthickness, vE=None, elmE=None,
                  coef=None, density=1.0, nu=0.5, kshear=5.0 / 6.0, mass_form="cpu", bdy_u=0.0,
-                 store_values=True, device=None):
+                 store_values=True, storage="sym", device=None):
+        if storage not in ("sym", "full"):
+            raise ValueError("storage must be 'sym' or 'full'")
+        self.storage = storage
         if not torch.cuda.is_available():
             raise RuntimeError("MembraneSystem needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.lib()
@@ -206,20 +212,34 @@ class MembraneSystem:
                                           ptr(self.vthick), self.thick_sstride, None, ptr(theta), 1,
                                           ptr(self.coef)))
         self.ave_E = torch.exp(theta)
-        if self.values is not None:
+        if self.values is not None or self.uvalues is not None:
             self.assemble_values()
 
     def assemble_values(self):
-        """Form (or re-form) the stored block values ``[nnzb][9][S]`` from ``coef``."""
+        """Form (or re-form) the stored block values from ``coef``: the upper blocks ``uvalues``
+        with symmetric storage, the full ``values [nnzb][9][S]`` with full storage (and the upper
+        blocks as well when they exist)."""
         S = self.n_smp
-        if self.values is None:
-            self.values = torch.empty((self.nnzb, 9, S), dtype=torch.float64, device=self.device)
-        check(self.lib.cvfes_assemble_bcsr(self.stream, self.nnzb, S, ptr(self.cptr), ptr(self.celem),
-                                           ptr(self.khat), ptr(self.coef), ptr(self.values)))
-        if self.uvalues is not None:
+        if self.storage == "full":
+            if self.values is None:
+                self.values = torch.empty((self.nnzb, 9, S), dtype=torch.float64, device=self.device)
+            check(self.lib.cvfes_assemble_bcsr(self.stream, self.nnzb, S, ptr(self.cptr),
+                                               ptr(self.celem), ptr(self.khat), ptr(self.coef),
+                                               ptr(self.values)))
+        if self.storage == "sym" or self.uvalues is not None:
             self.assemble_upper()
         if hasattr(self, "_sys"):
             self._sys = self._make_struct()
+
+    def set_storage(self, storage):
+        """Switch the stored values to ``"sym"`` (upper blocks only; the full values are dropped)
+        or ``"full"`` (every block; existing upper blocks are kept for the symmetric variants)."""
+        if storage not in ("sym", "full"):
+            raise ValueError("storage must be 'sym' or 'full'")
+        self.storage = storage
+        if storage == "sym":
+            self.values = None
+        self.assemble_values()
 
     def assemble_upper(self):
         """Form (or re-form) the symmetric storage ``uvalues [nnzu][9][S]`` (upper blocks only) used
@@ -497,9 +517,17 @@ class MembraneSystem:
 
     # ------------------------------------------------------------------------------------------
     def values_reference_layout(self):
-        """Stored values as the reference's ``[nnzb, S, 3, 3]`` (host numpy)."""
-        v = self.values.cpu().numpy()
-        return np.ascontiguousarray(v.reshape(self.nnzb, 3, 3, self.n_smp).transpose(0, 3, 1, 2))
+        """Stored values as the reference's ``[nnzb, S, 3, 3]`` (host numpy); with symmetric
+        storage every lower block is its mirror transposed."""
+        if self.values is not None:
+            v = self.values.cpu().numpy()
+            return np.ascontiguousarray(v.reshape(self.nnzb, 3, 3, self.n_smp).transpose(0, 3, 1, 2))
+        u = self.uvalues.cpu().numpy().reshape(self.nnzu, 3, 3, self.n_smp)
+        blk = u[self.maps.vblk]
+        row = np.repeat(np.arange(self.n_nodes), np.diff(self.indptr_h))
+        low = self.indices_h < row
+        blk[low] = blk[low].transpose(0, 2, 1, 3)
+        return np.ascontiguousarray(blk.transpose(0, 3, 1, 2))
 
     def lumped_mass_dofs(self):
         """Lumped mass as the reference's ``[ndof, S]`` (host numpy)."""

@@ -332,7 +332,10 @@ typedef struct cvfes_update {
  * nsteps steps of the updated-geometry mode: as cvfes_step, with the stiffness and the
  * unit-traction load those of the geometry X + ug (never stored: every step forms each element's
  * frame on the moved geometry in registers); lumped mass, coefficients and traction fields stay
- * as assembled.  Needs sys->nptr/nent/coef/lm.
+ * as assembled.  Needs sys->nptr/nent/coef/lm.  sys->variant selects the kernel form:
+ * CVFES_VARIANT_GENERIC divided frame normalisation, one sample per lane; CVFES_VARIANT_ROWLOAD1
+ * reciprocal normalisation (within an ulp), one sample per lane; CVFES_VARIANT_ROWLOAD and AUTO
+ * reciprocal normalisation, two samples per lane for even sample counts.
  * Replaces: SolidSolver.Solve with OptimizedAssembleUpdate (physicsSolver.py:263-324;
  *   optimizedSolidAssemble.pyx:710-819) for stiffness_update_interval = interval; the K(X+u) and
  *   follower-load products of assemble_K_P (physicsSolverSDampGPUs.cl:232-391) for interval 1.

@@ -29,14 +29,20 @@ from cvfes_amd.mesh import tube_geometry
 pytestmark = pytest.mark.gpu
 
 
-def build(nt, nz, R, L, S, seed, store=True):
+def build(nt, nz, R, L, S, seed, store=True, storage="full"):
     from cvfes_amd.system import MembraneSystem
     nodes, el, bdy = tube_geometry(nt, nz, R, L)
     rng = np.random.default_rng(seed)
     theta = np.log(7e6) + 0.1 * rng.standard_normal((el.shape[0], S))
     sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(nodes.shape[0], 0.07),
-                          elmE=np.exp(theta), density=1.0, nu=0.5, store_values=store)
+                          elmE=np.exp(theta), density=1.0, nu=0.5, store_values=store,
+                          storage=storage)
     return nodes, el, bdy, theta, sysm
+
+
+def method_storage(method):
+    """``stored-sym``: the stored operator kept as upper blocks only."""
+    return ("stored", "sym") if method == "stored-sym" else (method, "full")
 
 
 def sine(n, dt):
@@ -50,11 +56,14 @@ def sine(n, dt):
     ("matfree", 0.0, 25.0, 1, True, 4),
     ("stored", 1e-7, 10.0, 20, False, 100),
     ("matfree", 1e-7, 0.0, 9, True, 64),
+    ("stored-sym", 1e-7, 10.0, 11, True, 4),
+    ("stored-sym", 2e-7, 0.0, 20, False, 128),
 ])
 def test_adjoint_matches_torch_autograd(require_gpu, method, beta, damp, C, u0, S):
     from cvfes_amd.adjoint import AdjointProblem, EnsembleLoss
     from oracle.torch_adjoint import TorchEnsemble
-    nodes, el, bdy, theta, sysm = build(16, 6, 0.3, 1.0, S, 3)
+    method, storage = method_storage(method)
+    nodes, el, bdy, theta, sysm = build(16, 6, 0.3, 1.0, S, 3, storage=storage)
     nsteps, dt = 60, 4e-6
     p = sine(nsteps, dt)
     obs = [12, 24, 36, 48, 60]
@@ -84,7 +93,8 @@ def test_adjoint_matches_torch_autograd(require_gpu, method, beta, damp, C, u0, 
     assert np.max(np.abs(g - gref)) / np.max(np.abs(gref)) < 1e-9
 
 
-@pytest.mark.parametrize("method,S", [("stored", 8), ("matfree", 8), ("stored", 5), ("matfree", 66)])
+@pytest.mark.parametrize("method,S", [("stored", 8), ("matfree", 8), ("stored", 5), ("matfree", 66),
+                                      ("stored-sym", 8), ("stored-sym", 5)])
 def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
     """The pipelined sweep (recomputation of a segment fused with the adjoint steps of the next)
     gives the gradient of the separate kernels bit for bit, with beta, damping, an initial
@@ -92,7 +102,8 @@ def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
     divide the step count, with one and with two checkpoint levels, also with the kernels capped
     at 4 waves per SIMD; and it does fuse steps."""
     from cvfes_amd.adjoint import AdjointProblem
-    nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 7)
+    method, storage = method_storage(method)
+    nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 7, storage=storage)
     nsteps, dt = 75, 4e-6
     p = sine(nsteps, dt)
     obs = [8, 30, 31, 64, 75]

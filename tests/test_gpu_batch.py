@@ -26,7 +26,7 @@ SPECS = [  # ntheta, nz, R, S, method, dt, beta, damp
 ]
 
 
-def make_ensemble(nsteps, seed=0):
+def make_ensemble(nsteps, seed=0, storage="full"):
     from cvfes_amd.system import MembraneSystem
     rng = np.random.default_rng(seed)
     systems, loads = [], []
@@ -34,7 +34,7 @@ def make_ensemble(nsteps, seed=0):
         nodes, el, bdy = tube_geometry(nt, nz, R, 10 * R)
         E = 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
         s = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(nodes.shape[0], 0.07), elmE=E,
-                           density=1.0, nu=0.5, store_values=(method == "stored"))
+                           density=1.0, nu=0.5, store_values=(method == "stored"), storage=storage)
         s.set_variant("generic")
         systems.append(s)
         t = np.arange(nsteps) * dt
@@ -46,16 +46,17 @@ def states(systems):
     return [s.u.cpu().numpy().copy() for s in systems]
 
 
-@pytest.mark.parametrize("vw_hint,cap,waves", [(0, False, 0), (1, False, 0), (2, False, 0),
-                                               (2, True, 0), (0, True, 0), (2, False, 4),
-                                               (0, True, 7)])
-def test_batched_equals_per_mesh(require_gpu, vw_hint, cap, waves):
+@pytest.mark.parametrize("vw_hint,cap,waves,storage", [
+    (0, False, 0, "full"), (1, False, 0, "full"), (2, False, 0, "full"), (2, True, 0, "full"),
+    (0, True, 0, "full"), (2, False, 4, "full"), (0, True, 7, "full"), (0, False, 0, "sym"),
+    (1, True, 0, "sym")])
+def test_batched_equals_per_mesh(require_gpu, vw_hint, cap, waves, storage):
     """Every lane-width policy (two samples per lane where even / one everywhere / two only on
-    wave-uniform rows), with or without the register and waves-per-SIMD caps, gives the per-mesh
-    result bit for bit."""
+    wave-uniform rows), with or without the register and waves-per-SIMD caps, and the stored
+    meshes in full or symmetric storage, gives the per-mesh result bit for bit."""
     from cvfes_amd.batch import BatchedEnsemble
     nsteps = 40
-    systems, loads = make_ensemble(nsteps)
+    systems, loads = make_ensemble(nsteps, storage=storage)
     cols = list(zip(*SPECS))
     ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
                           betas=list(cols[6]), damps=list(cols[7]))

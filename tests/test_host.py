@@ -245,10 +245,13 @@ def test_step_kernel_resolution_without_gpu():
     V = _lib.VARIANTS
     dummy = ctypes.c_void_p(16)                 # never dereferenced: resolution reads the struct only
 
-    def kern(S, variant, method, uvalues=False, lds=False, limits=(7, 2, 6), colors=False):
+    def kern(S, variant, method, uvalues=False, lds=False, limits=(7, 2, 6), colors=False,
+             full=True):
         sysd = _lib.CvfesSystem(n_nodes=4, n_smp=S, nnzb=4, variant=V[variant],
                                 max_row_blocks=limits[0], max_offdiag_contrib=limits[1],
                                 max_diag_contrib=limits[2])
+        if full:
+            sysd.values = dummy
         if uvalues:
             sysd.uvalues = sysd.vblk = dummy
         if lds:
@@ -278,6 +281,14 @@ def test_step_kernel_resolution_without_gpu():
     assert kern(128, "generic", "stored") == V["generic"]
     assert kern(128, "ebe", "matfree") == V["generic"]          # no element colouring
     assert kern(128, "ebe", "matfree", colors=True) == V["ebe"]
+    # symmetric storage only (no full values): the symmetric kernel, or the generic one
+    for v in ("auto", "rowload_nt", "rowload1", "sym_ntl", "lds"):
+        assert kern(128, v, "stored", uvalues=True, full=False) == V["sym_ntl"], v
+    assert kern(128, "sym", "stored", uvalues=True, full=False) == V["sym"]
+    assert kern(128, "generic", "stored", uvalues=True, full=False) == V["generic"]
+    assert kern(100, "auto", "stored", uvalues=True, full=False) == V["generic"]
+    assert kern(128, "auto", "stored", uvalues=True, full=False, limits=(7, 3, 6)) == V["generic"]
+    assert kern(128, "auto", "matfree", uvalues=True, full=False) == V["generic"]
     assert kern(97, "ebe", "matfree", colors=True) == V["ebe"]
     assert kern(128, "ebe", "stored", colors=True) == V["rowload_nt"]
     assert lib.cvfes_step_kernel(None, 0) == -1
