@@ -101,6 +101,8 @@ _SIGNATURES = {
     "cvfes_bcsr_spmv": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp]),
     "cvfes_initial_state": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp,
                                    ctypes.POINTER(CvfesLoad), _f64, _f64, _f64, _vp, _vp, _vp]),
+    "cvfes_initial_state_static": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp,
+                                          ctypes.POINTER(CvfesLoad), _f64, _f64, _f64, _vp, _vp]),
     "cvfes_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, ctypes.POINTER(_vp),
                           ctypes.POINTER(CvfesLoad), _i64, _f64, _f64, _f64]),
     "cvfes_adjoint_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, _vp, _vp, _vp, _vp, _vp,
@@ -128,6 +130,9 @@ _SIGNATURES = {
 }
 
 EXPORTED = tuple(_SIGNATURES)
+
+# Doubles in the static device scratch of cvfes_initial_state_static (CVFES_STATIC_SCRATCH_DOUBLES).
+STATIC_SCRATCH_DOUBLES = 8 << 20
 
 _LIB = None
 

@@ -15,6 +15,8 @@
 //                   contributions (sample-independent khat, scalar loads; coef [E][S] is cached
 //                   across the 3 rows that touch an element), or, in the EBE variant, element by
 //                   element over element colours.
+#include <mutex>
+
 #include "rowops.h"
 
 namespace cvfes {
@@ -849,6 +851,59 @@ extern "C" int cvfes_initial_state(void *stream, const cvfes_system_t *sys, int 
                                         "cvfes_initial_state"))
         return rc;
     CVFES_REQUIRE(work, "cvfes_initial_state: null work buffer");
+    return initial_state_impl(static_cast<hipStream_t>(stream), sys, method, u0, du0, load, dt,
+                              beta, damp, u, up, work);
+}
+
+// Device scratch of cvfes_initial_state_static: the product K (u0 + beta du0) between its two
+// launches.  Every GPU the library runs on holds its own copy of this array until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device memory: kernels get the
+// per-device address from hipGetSymbolAddress (scratch_device_ptr) and never the name itself.
+static __device__ __align__(256) double scratch[CVFES_STATIC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double *scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's copy of `scratch`, looked up once per device (with that device
+// current) and kept.
+int scratch_device_ptr(double **out, const char *who) {
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return fail(CVFES_ERR_HIP, "%s: hipGetDevice: %s", who, hipGetErrorString(err));
+    CVFES_REQUIRE(dev >= 0 && dev < kMaxDevices, "%s: device %d beyond the %d supported", who, dev,
+                  kMaxDevices);
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return fail(CVFES_ERR_HIP, "%s: hipGetSymbolAddress(scratch) on device %d: %s", who, dev,
+                        hipGetErrorString(err));
+        scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = scratch_ptr[dev];
+    return CVFES_OK;
+}
+
+}  // namespace
+
+extern "C" int cvfes_initial_state_static(void *stream, const cvfes_system_t *sys, int method,
+                                          const double *u0, const double *du0,
+                                          const cvfes_load_t *load, double dt, double beta,
+                                          double damp, double *u, double *up) {
+    const char *who = "cvfes_initial_state_static";
+    if (int rc = validate_initial_state(sys, method, load, u0, du0, dt, u, up, who)) return rc;
+    const int64_t need = 3 * sys->n_nodes * sys->n_smp;
+    CVFES_REQUIRE(need <= CVFES_STATIC_SCRATCH_DOUBLES,
+                  "%s: 3 x %lld nodes x %lld samples = %lld values exceed the static scratch of %lld "
+                  "(use cvfes_initial_state with a work buffer)",
+                  who, static_cast<long long>(sys->n_nodes), static_cast<long long>(sys->n_smp),
+                  static_cast<long long>(need), static_cast<long long>(CVFES_STATIC_SCRATCH_DOUBLES));
+    double *work = nullptr;
+    if (int rc = scratch_device_ptr(&work, who)) return rc;
     return initial_state_impl(static_cast<hipStream_t>(stream), sys, method, u0, du0, load, dt,
                               beta, damp, u, up, work);
 }

@@ -232,6 +232,25 @@ int cvfes_initial_state(void *stream, const cvfes_system_t *sys, int method, con
                         const double *du0, const cvfes_load_t *load, double dt, double beta,
                         double damp, double *u, double *up, double *work);
 
+/* Size of the static device scratch of cvfes_initial_state_static, in doubles (64 MiB). */
+#define CVFES_STATIC_SCRATCH_DOUBLES (INT64_C(8) << 20)
+
+/*
+ * cvfes_initial_state without the work buffer, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It allocates nothing: the product K (u0 + beta du0) goes to a
+ * static device array of CVFES_STATIC_SCRATCH_DOUBLES doubles that every GPU the library runs on
+ * holds until the process ends (its address is looked up once per device on the first call, with
+ * that device current; the stream must belong to the current device).  Systems with
+ * 3 * n_nodes * n_smp above that limit are rejected with CVFES_ERR_ARG.  All calls on one GPU share
+ * the array, so calls that overlap on one GPU (concurrent streams or threads) race; calls on one
+ * stream do not.  Nothing in the array needs to start at zero (the product writes every entry
+ * before it is read).  Results equal cvfes_initial_state's bit for bit (same kernels).
+ */
+int cvfes_initial_state_static(void *stream, const cvfes_system_t *sys, int method,
+                               const double *u0, const double *du0, const cvfes_load_t *load,
+                               double dt, double beta, double damp, double *u, double *up);
+
 /*
  * nsteps explicit central-difference steps with the stiffness product, Rayleigh stiffness
  * damping, mass-independent damping, load and Dirichlet values fused into one kernel per step:
