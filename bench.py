@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matfree", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward ratio of the differentiation workload")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -344,6 +346,14 @@ def main():
         print(f"[bench] cpu baseline {cres['value']:.1f} sample-steps/s on {threads} threads"
               + (f" ({one:.1f} on one)" if one else "") + "; "
               f"parity {out['parity_vs_cpu']['rel_max_err']:.2e}", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        # the differentiation workload's cost ratio (adjoint gradient / forward solve)
+        r = adjoint_measure("tube-192x256-s32", "stored")
+        out["adjoint_over_forward"] = {k: r[k] for k in (
+            "workload", "n_samples", "steps", "observations", "checkpoint_every", "ratio",
+            "forward_s", "adjoint_s", "ratio_separate_kernels", "fused_steps", "fused_equals_separate")}
+        print(f"[bench] adjoint / forward {r['ratio']:.2f} ({r['forward_s']:.3f} s forward)",
+              file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 
@@ -589,33 +599,37 @@ def run_updated(args):
     print(json.dumps(out), flush=True)
 
 
-def run_adjoint(args):
-    """Forward and adjoint gradient of the displacement misfit on the 49,152-node tube."""
+def adjoint_measure(case_name="tube-192x256-s32", method="stored"):
+    """Forward solve and adjoint gradient of the displacement misfit (20 observation times,
+    checkpoints every 100 steps), timed: the pipelined (fused) reverse sweep and the one with
+    separate kernels, whose gradients must agree bit for bit."""
     import torch
     from cvfes_amd.adjoint import AdjointProblem
     from cvfes_amd.configs import get_case
     from cvfes_amd.gmrf import sample_element_E
     from cvfes_amd.system import MembraneSystem
 
-    case = get_case(args.case if args.case != "tube-256x400-s128" else "tube-192x256-s32")
+    case = get_case(case_name)
     nodes, el, bdy = case.geometry()
     N, S = nodes.shape[0], case.n_smp
-    theta = np.log(case.sample_E(el).E)
-    sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=np.exp(theta),
+    gm = sample_element_E(el, S, seed=case.seed, E0=case.E0, cov=case.cov, device="cuda")
+    theta = np.log(gm.E)
+    sysm = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(N, case.thickness), elmE=gm.E,
                           density=case.density, nu=case.nu)
     nsteps, dt = case.nsteps, case.dt
     load = sysm.loads(case.pressures(0, nsteps))
     obs = list(range(nsteps // 20, nsteps + 1, nsteps // 20))
-    theta_obs = np.log(sample_element_E(el, S, seed=case.seed + 100).E)
+    theta_obs = np.log(sample_element_E(el, S, seed=case.seed + 100, E0=case.E0, cov=case.cov,
+                                        tau=gm.tau, device="cuda").E)
     sysm.set_log_modulus(torch.as_tensor(theta_obs).cuda())
     u0 = np.zeros(3 * N)
-    sysm.initial_state(u0, u0, load, dt)
+    sysm.initial_state(u0, u0, load, dt, method=method)
     states, done = [], 0
     for k in obs:
-        sysm.step(load, dt, offset=done, nsteps=k - done)
+        sysm.step(load, dt, method=method, offset=done, nsteps=k - done)
         done = k
         states.append(sysm.u.clone())
-    prob = AdjointProblem(sysm, load, nsteps, dt, method=args.method, obs_steps=obs,
+    prob = AdjointProblem(sysm, load, nsteps, dt, method=method, obs_steps=obs,
                           u_obs=torch.stack(states), checkpoint_every=100)
     prob.set_theta(torch.as_tensor(theta).cuda())
     prob.forward()
@@ -639,22 +653,31 @@ def run_adjoint(args):
     g_sep = prob.backward()
     torch.cuda.synchronize()
     adj_sep = time.perf_counter() - t3
+    return {"workload": case.name, "n_nodes": N, "n_samples": S, "steps": nsteps,
+            "observations": len(obs), "checkpoint_every": 100, "operator": method,
+            "storage": sysm.storage if method == "stored" else None,
+            "forward_s": fwd, "adjoint_s": adj, "ratio": adj / fwd,
+            "forward_sample_steps_per_s": S * nsteps / fwd,
+            "gradient_max_abs": float(g.abs().max()), "fused_steps": fused_steps,
+            "fusion_timing_ms": fusion_timing, "adjoint_separate_kernels_s": adj_sep,
+            "ratio_separate_kernels": adj_sep / fwd, "fused_equals_separate": bool(torch.equal(g, g_sep))}
+
+
+def run_adjoint(args):
+    """Forward and adjoint gradient of the displacement misfit on the 49,152-node tube."""
+    case = args.case if args.case != "tube-256x400-s128" else "tube-192x256-s32"
+    r = adjoint_measure(case, args.method)
     out = {
         "metric": "adjoint gradient time / forward solve time (49k-node tube, 32 samples, 2,000 steps)",
-        "value": adj / fwd, "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": nsteps,
-        "ms_per_step": 1e3 * (fwd + adj) / nsteps, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded tube and GMRF samples",
-        "config": {"workload": case.name, "n_nodes": N, "n_samples": S, "observations": len(obs),
-                   "checkpoint_every": 100, "operator": args.method},
-        "forward_s": fwd, "adjoint_s": adj,
-        "forward_sample_steps_per_s": S * nsteps / fwd,
-        "gradient_max_abs": float(g.abs().max()),
-        "fused_steps": fused_steps, "fusion_timing_ms": fusion_timing,
-        "adjoint_separate_kernels_s": adj_sep, "ratio_separate_kernels": adj_sep / fwd,
-        "fused_equals_separate": bool(torch.equal(g, g_sep)),
+        "value": r["ratio"], "unit": "ratio", "n_gpus": 1, "steps": r["steps"], "warmup": r["steps"],
+        "ms_per_step": 1e3 * (r["forward_s"] + r["adjoint_s"]) / r["steps"], "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded tube and GMRF samples",
+        "config": {k: r[k] for k in ("workload", "n_nodes", "n_samples", "observations",
+                                     "checkpoint_every", "operator", "storage")},
     }
+    out.update({k: v for k, v in r.items() if k not in out["config"]})
     print(json.dumps(out), flush=True)
-
 
 
 def run_combined(args):

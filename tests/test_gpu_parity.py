@@ -19,6 +19,9 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+_CASE_IDS = iter(range(1 << 30))
+
+
 class Case:
     """Inputs for one ensemble shared by the GPU system and the oracle."""
 
@@ -27,6 +30,7 @@ class Case:
         from cvfes_amd.system import MembraneSystem
 
         c = get_case(name)
+        self.uid = next(_CASE_IDS)       # oracle cache key (id() values are reused after collection)
         self.case = c
         self.S = n_smp or c.n_smp
         self.nodes, self.el, self.bdy = c.geometry()
@@ -103,7 +107,7 @@ def _run_both(case, nsteps, method, beta=0.0, damp=0.0, dt=None, record=()):
         sysm.step(p, dt, beta=beta, damp=damp, method=method, offset=done, nsteps=k - done)
         done = k
         out[k] = sysm.u.cpu().numpy().reshape(-1, case.S)
-    key = (id(case), nsteps, beta, damp, dt, tuple(sorted(record)))
+    key = (case.uid, nsteps, beta, damp, dt, tuple(sorted(record)))
     if key not in _ORACLE_CACHE:
         _ORACLE_CACHE[key] = so.run(case.indptr, case.indices, case.K, case.LM, case.f1, p, u0, u0,
                                     dt, beta=beta, damp=damp, bdy_dofs=case.bdy_dofs, bdy_u=0.0,
