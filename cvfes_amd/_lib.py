@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -54,7 +54,8 @@ class CvfesSystem(ctypes.Structure):
         ("variant", _i64), ("nptr", _vp), ("nent", _vp), ("cslot", _vp), ("max_node_elems", _i64),
         ("uvalues", _vp), ("vblk", _vp), ("nnzu", _i64), ("row_order", _vp),
         ("color_ptr", _vp), ("color_elems", _vp), ("elems", _vp), ("cpos", _vp), ("n_colors", _i64),
-        ("max_color_elems", _i64), ("waves_per_simd", _i64),
+        ("max_color_elems", _i64), ("waves_per_simd", _i64), ("row_records", _vp),
+        ("record_width", _i64),
     ]
 
 

@@ -185,19 +185,29 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
 // about halved.  Value loads: NT = 0 plain, 1 all non-temporal, 2 non-temporal for the mirrored
 // (second) reads only.  Loads are issued for the whole row before the arithmetic, as in the
 // row-load kernel.
-template <bool BETA, bool DAMP, int NT, int MAXB, int VW>
+// REC: the row's structure comes from its record (`rec`, see cvfes_system_t.row_records): one
+// load chain shorter than row order -> indptr -> indices/vblk, which matters at one wave per SIMD.
+template <bool BETA, bool DAMP, int NT, int MAXB, int VW, bool REC>
 __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *__restrict__ uvalues,
-                                                     const int32_t *__restrict__ vblk, int64_t step) {
+                                                     const int32_t *__restrict__ vblk,
+                                                     const int32_t *__restrict__ rec, int64_t step) {
     typedef typename VecT<VW>::type V;
+    constexpr int RECW = 2 + 2 * MAXB;
     int32_t row;
     int64_t s;
     bool active;
     if (!row_and_sample<true, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
-    if (a.order) row = a.order[row];
+    const int32_t *r = nullptr;
+    if constexpr (REC) {
+        r = rec + static_cast<int64_t>(row) * RECW;
+        row = r[0];
+    } else if (a.order) {
+        row = a.order[row];
+    }
     const RowArgs &k = a.k;
     const int64_t S = k.n_smp;
-    const int32_t p0 = k.indptr[row];
-    const int32_t nb = k.indptr[row + 1] - p0;
+    const int32_t p0 = REC ? 0 : k.indptr[row];
+    const int32_t nb = REC ? r[1] : k.indptr[row + 1] - p0;
     V uo[3], po[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -211,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
         for (int b = 0; b < MAXB; ++b) {
             const bool valid = b < nb;
             const int32_t pb = p0 + (valid ? b : nb - 1);
-            const int32_t col = k.indices[pb];
+            const int32_t col = REC ? r[2 + b] : k.indices[pb];
             const bool tr = col < row;
             V w[3];
 #pragma unroll
@@ -225,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
                     w[j] = xu;
                 }
             }
-            const double *base = uvalues + static_cast<int64_t>(vblk[pb]) * 9 * S + s;
+            const double *base = uvalues + static_cast<int64_t>(REC ? r[2 + MAXB + b] : vblk[pb]) * 9 * S + s;
             V kv[9];
             if (NT == 2 && tr) {
 #pragma unroll
@@ -462,20 +472,30 @@ void launch_lds_bd(hipStream_t st, const StepArgs &a, const LdsArgs &l, int64_t 
     }
 }
 
-template <int NT, int MAXB>
-void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step, bool beta,
-                bool damp) {
+template <int NT, int MAXB, bool REC>
+void launch_sym_rec(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step,
+                    bool beta, bool damp) {
     const int64_t work = a.k.n_nodes * lanes_per_row<true, 2>(a.k.n_smp);
     const dim3 grid(grid_for(work)), block(kBlock);
     const double *uv = sys->uvalues;
     const int32_t *vb = sys->vblk;
+    const int32_t *rc = sys->row_records;
     if (beta) {
-        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
-        else hipLaunchKernelGGL((k_step_sym<true, false, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
+        if (damp) hipLaunchKernelGGL((k_step_sym<true, true, NT, MAXB, 2, REC>), grid, block, a.occ_lds, st, a, uv, vb, rc, step);
+        else hipLaunchKernelGGL((k_step_sym<true, false, NT, MAXB, 2, REC>), grid, block, a.occ_lds, st, a, uv, vb, rc, step);
     } else {
-        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
-        else hipLaunchKernelGGL((k_step_sym<false, false, NT, MAXB, 2>), grid, block, a.occ_lds, st, a, uv, vb, step);
+        if (damp) hipLaunchKernelGGL((k_step_sym<false, true, NT, MAXB, 2, REC>), grid, block, a.occ_lds, st, a, uv, vb, rc, step);
+        else hipLaunchKernelGGL((k_step_sym<false, false, NT, MAXB, 2, REC>), grid, block, a.occ_lds, st, a, uv, vb, rc, step);
     }
+}
+
+template <int NT, int MAXB>
+void launch_sym(hipStream_t st, const cvfes_system_t *sys, const StepArgs &a, int64_t step, bool beta,
+                bool damp) {
+    if (sys->row_records && sys->record_width == 2 + 2 * MAXB)
+        launch_sym_rec<NT, MAXB, true>(st, sys, a, step, beta, damp);
+    else
+        launch_sym_rec<NT, MAXB, false>(st, sys, a, step, beta, damp);
 }
 
 template <int VW, int MAXE>

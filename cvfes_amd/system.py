@@ -136,6 +136,9 @@ class MembraneSystem:
         self._nodes_h = nodes
         self._elems_h = np.asarray(elements, np.int64)
         self.row_order = None
+        self._order_h = None                          # host copy of the visiting order (None: natural)
+        self.row_records = None                       # symmetric kernels' row records (on demand)
+        self.record_width = 0
         self.color_ptr = self.color_elems = None      # element colouring (EBE variant), on demand
         self.n_colors = self.max_color_elems = 0
         self.waves_per_simd = 0                       # occupancy cap of the step kernels (0 = none)
@@ -250,8 +253,41 @@ class MembraneSystem:
         check(self.lib.cvfes_assemble_upper(self.stream, self.nnzu, S, ptr(self.ublk), ptr(self.cptr),
                                             ptr(self.celem), ptr(self.khat), ptr(self.coef),
                                             ptr(self.uvalues)))
+        if self.row_records is None:
+            self._build_records()
         if hasattr(self, "_sys"):
             self._sys = self._make_struct()
+
+    def _record_slots(self):
+        """Block slots of the symmetric kernels for this structure (as the C side picks them), or
+        0 when they do not apply."""
+        if self.max_offdiag_contrib > 2:
+            return 0
+        for maxb in (7, 10):
+            if self.max_row_blocks <= maxb and self.max_diag_contrib <= maxb - 1:
+                return maxb
+        return 0
+
+    def _build_records(self):
+        """Row records of the symmetric kernels in the current visiting order: row id, block count,
+        the blocks' columns and upper-storage entries, padded with the row's last block."""
+        maxb = self._record_slots()
+        if maxb == 0:
+            self.row_records, self.record_width = None, 0
+            return
+        ip = self.indptr_h.astype(np.int64)
+        rows = np.arange(self.n_nodes) if self._order_h is None else self._order_h.astype(np.int64)
+        p0 = ip[rows]
+        nb = ip[rows + 1] - p0
+        slot = p0[:, None] + np.minimum(np.arange(maxb)[None, :], np.maximum(nb - 1, 0)[:, None])
+        slot = np.minimum(slot, max(self.nnzb - 1, 0))
+        rec = np.empty((self.n_nodes, 2 + 2 * maxb), np.int32)
+        rec[:, 0] = rows
+        rec[:, 1] = nb
+        rec[:, 2:2 + maxb] = self.indices_h[slot]
+        rec[:, 2 + maxb:] = self.maps.vblk[slot]
+        self.row_records = _dev(rec, torch.int32, self.device)
+        self.record_width = 2 + 2 * maxb
 
     def release_values(self):
         """Drop the stored values (matrix-free runs only need coef and khat)."""
@@ -272,7 +308,8 @@ class MembraneSystem:
             nnzu=self.nnzu, row_order=ptr(self.row_order), color_ptr=ptr(self.color_ptr),
             color_elems=ptr(self.color_elems), elems=ptr(self.elems), cpos=ptr(self.cpos),
             n_colors=self.n_colors, max_color_elems=self.max_color_elems,
-            waves_per_simd=self.waves_per_simd)
+            waves_per_simd=self.waves_per_simd, row_records=ptr(self.row_records),
+            record_width=self.record_width)
 
     def set_variant(self, variant):
         """Select the step kernel variant (names in ``_lib.VARIANTS``); the symmetric ones form the
@@ -327,6 +364,9 @@ class MembraneSystem:
             if perm.shape != (self.n_nodes,) or not np.array_equal(np.sort(perm), np.arange(self.n_nodes)):
                 raise ValueError("row order must be a permutation of the nodes")
             self.row_order = _dev(perm.astype(np.int32), torch.int32, self.device)
+        self._order_h = None if self.row_order is None else perm.astype(np.int32)
+        if self.uvalues is not None:
+            self._build_records()
         self._sys = self._make_struct()
 
     def set_occupancy(self, waves_per_simd=0):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 10
+#define CVFES_ABI_VERSION 11
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -93,6 +93,15 @@ typedef struct cvfes_system {
      * gathered neighbour data in the XCD's L2 when the rows are large (many samples, long rings);
      * the results do not depend on it. */
     int64_t waves_per_simd;
+    /* Row records of the symmetric kernels (may be NULL): for every visited position v (in the
+     * order of row_order, or natural), record_width int32 values: the row id, its block count,
+     * then the columns of its blocks and their upper-storage entries (vblk), MAXB of each, padded
+     * with the row's last block, where MAXB = (record_width - 2) / 2 is the kernel's block-slot
+     * count (7 or 10, as the row-load kernels choose it).  One scalar load then gives a row its
+     * whole structure.  The kernels fall back to indptr/indices/vblk when the width does not
+     * match; the results are the same either way. */
+    const int32_t *row_records;
+    int64_t record_width;
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */

@@ -255,6 +255,28 @@ def test_row_order_does_not_change_result(case128, method, variant):
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
+@pytest.mark.parametrize("variant", ["sym", "sym_nt", "sym_ntl"])
+def test_row_records_do_not_change_result(require_gpu, variant):
+    """The symmetric kernels reading each row's structure from its record give the states of the
+    kernels reading indptr/indices/vblk, bit for bit, in the natural and a strip order."""
+    c = Case("tube-64x32-s16", n_smp=128, seed=13, storage="sym")
+    s = c.sys
+    assert s.row_records is not None and s.record_width == 16
+    s.set_variant(variant)
+    for order in ("natural", "strips"):
+        s.set_row_order(order, width=16)
+        outs = []
+        for records in (True, False):
+            if not records:
+                s.row_records, s.record_width = None, 0
+                s._sys = s._make_struct()
+            out, snaps = _run_both(c, 40, "stored", beta=1e-6, damp=30.0, dt=3e-6, record=(40,))
+            outs.append(out[40])
+            assert rel_err(out[40], snaps[40]) < REL_U
+        assert np.array_equal(outs[0], outs[1])
+        s._build_records()
+
+
 @pytest.mark.parametrize("method,variant", [("stored", "rowload_nt"), ("stored", "sym_ntl"),
                                             ("stored", "generic"), ("matfree", "generic")])
 def test_occupancy_cap_does_not_change_result(case128, method, variant):
