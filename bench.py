@@ -199,6 +199,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matfree", action="store_true")
+    ap.add_argument("--adjoint-block", type=int, default=0,
+                    help="combined workload: reverse sweep by sample blocks of this size (0 = two levels)")
+    ap.add_argument("--adjoint-every", type=int, default=0, help="combined workload: checkpoint interval")
+    ap.add_argument("--adjoint-host", action="store_true", help="combined workload: checkpoints in host memory")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward ratio of the differentiation workload")
     args = ap.parse_args()
@@ -732,9 +736,17 @@ def run_combined(args):
         done = k
         u_obs[i].copy_(sysm.u)
     log(f"observations ({time.perf_counter() - t_start:.0f} s)")
-    C1, C2 = 200, 20
+    if args.adjoint_block:
+        # one checkpoint level, the reverse sweep by sample blocks so that a block's recomputed
+        # segment fits the device beside the checkpoints
+        C, C2, B = args.adjoint_every or 125, None, args.adjoint_block
+    else:
+        # two levels (pairs every 200 steps, and every 20 inside a segment): the whole ensemble's
+        # states are 3.1 GB each, so one level (2,000/C pairs + C states) does not fit 288 GB
+        C, C2, B = args.adjoint_every or 200, 20, None
     prob = AdjointProblem(sysm, load, nsteps, dt, beta=beta, method="matfree", obs_steps=obs,
-                          u_obs=u_obs, checkpoint_every=C1, inner_every=C2)
+                          u_obs=u_obs, checkpoint_every=C, inner_every=C2, sample_block=B,
+                          host_checkpoints=args.adjoint_host)
     prob.set_theta(torch.as_tensor(theta).cuda())
     torch.cuda.reset_peak_memory_stats()
     torch.cuda.synchronize()
@@ -757,7 +769,9 @@ def run_combined(args):
         "data": "synthetic: seeded stenosed tube and GMRF samples",
         "config": {"workload": case.name, "n_nodes": N, "n_elements": E, "nnzb": sysm.nnzb,
                    "n_samples": S, "dt": dt, "beta": beta, "observations": len(obs),
-                   "checkpoints": [C1, C2], "adjoint_operator": "matfree"},
+                   "checkpoint_every": C, "inner_every": C2, "sample_block": B,
+                   "checkpoints_in": "host memory" if args.adjoint_host else "device memory",
+                   "adjoint_operator": "matfree"},
         "step_operators": per_step, "stored_vs_matfree_rel_max_err": agree,
         "forward_s": fwd, "adjoint_s": adj, "adjoint_over_forward": adj / fwd,
         "fused_steps": prob.stats.get("fused_steps", 0),

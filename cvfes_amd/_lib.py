@@ -13,7 +13,7 @@ from pathlib import Path
 import torch  # noqa: F401  (load torch's HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libcvfes_mi355x.so"
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 METHOD_STORED = 0
 METHOD_MATFREE = 1
@@ -55,7 +55,7 @@ class CvfesSystem(ctypes.Structure):
         ("uvalues", _vp), ("vblk", _vp), ("nnzu", _i64), ("row_order", _vp),
         ("color_ptr", _vp), ("color_elems", _vp), ("elems", _vp), ("cpos", _vp), ("n_colors", _i64),
         ("max_color_elems", _i64), ("waves_per_simd", _i64), ("row_records", _vp),
-        ("record_width", _i64),
+        ("record_width", _i64), ("minv", _vp),
     ]
 
 
@@ -110,6 +110,7 @@ _SIGNATURES = {
                                   _vp, _f64, _f64, _f64, _int]),
     "cvfes_grad_accum": (_int, [_vp, ctypes.POINTER(CvfesSystem), _vp, _vp, _vp, _vp, _vp, _f64,
                                 _f64, _f64, _vp]),
+    "cvfes_inverse_mass": (_int, [_vp, _i64, _vp, _vp]),
     "cvfes_step_kernel": (_i64, [ctypes.POINTER(CvfesSystem), _int]),
     "cvfes_adjoint_fused_step": (_int, [_vp, ctypes.POINTER(CvfesSystem), _int, ctypes.POINTER(_vp),
                                         ctypes.POINTER(CvfesLoad), _vp, _vp, _vp, _vp, _vp, _vp,

@@ -15,7 +15,14 @@ only ``C2`` states are held at a time, so memory falls from ``2 n/C + C`` to
 pipelined: the segment before the current one is recomputed during the current one's adjoint
 steps, a recomputed step and an adjoint step per fused kernel, in the buffers the current one
 releases.  Observations may live in host memory (copied to the device when used).  Recomputation
-replays the same arithmetic, so the gradient does not depend on the checkpoint scheme.  ``EnsembleLoss`` wraps it as a ``torch.autograd.Function`` of
+replays the same arithmetic, so the gradient does not depend on the checkpoint scheme.
+
+Samples do not interact, so the reverse sweep can also run sample block by sample block
+(``sample_block``): each block's recomputed segment then takes ``B/S`` of the memory, which lets a
+one-level scheme fit where the whole ensemble would need two levels (one forward pass less).  The
+checkpoints (one pair per segment and block) can be kept in host memory (``host_checkpoints``):
+a helper thread copies them out during the forward pass and back one pair ahead during the
+reverse sweep, on a stream of its own, while the main thread keeps the GPU busy.  ``EnsembleLoss`` wraps it as a ``torch.autograd.Function`` of
 ``theta = log E [E, S]``.
 
 All device work is in the native kernels; torch provides the buffers, the per-sample loss
@@ -23,6 +30,7 @@ reduction and the autograd plumbing.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import ctypes
 
 import numpy as np
@@ -54,11 +62,16 @@ class AdjointProblem:
         on the problem's own states first and keeps the faster form (the fused kernel carries
         two columns of states and loses where they overflow L2); the gradient is the same bit
         for bit either way.
+    sample_block : run the reverse sweep for blocks of this many samples at a time (matrix-free;
+        one checkpoint level; no fused kernels).  The gradient is that of the whole-ensemble
+        sweep bit for bit.
+    host_checkpoints : keep the checkpoints in (pageable) host memory, one pair per segment and
+        sample block, copied by a helper thread while the kernels run.
     """
 
     def __init__(self, system: MembraneSystem, load: LoadTables, nsteps, dt, beta=0.0, damp=0.0,
                  method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None,
-                 inner_every=None, fuse="auto"):
+                 inner_every=None, fuse="auto", sample_block=None, host_checkpoints=False):
         self.sys = system
         self.load = load
         self.nsteps = int(nsteps)
@@ -78,6 +91,29 @@ class AdjointProblem:
         self.du0 = np.zeros(3 * N) if du0 is None else np.asarray(du0, np.float64)
         self.checkpoints = {}
         self.stats = {}
+        S = system.n_smp
+        self.sample_block = None if sample_block is None else int(sample_block)
+        self.host_checkpoints = bool(host_checkpoints)
+        self._blocks = None
+        if self._blocked():
+            if self.sample_block is not None and not 1 <= self.sample_block <= S:
+                raise ValueError("sample_block must lie in 1..n_smp")
+            if self.C2 is not None:
+                raise ValueError("sample blocks and host checkpoints run one checkpoint level")
+            if method != "matfree" and self._ranges() != [(0, S)]:
+                raise ValueError("sample blocks run the matrix-free operator")
+            self.fuse = False
+            dev = system.device
+            self._copy_stream = torch.cuda.Stream(device=dev)
+            self._stage = [None, None]          # device staging of the checkpoint copies
+            self._stage_jobs = [None, None]     # helper-thread copies out of each staging slot
+            self._stage_i = 0
+            self._host = {}
+            if self.host_checkpoints:
+                self._worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+                for mark in range(0, self.nsteps, self.C):
+                    self._host[mark] = [tuple(torch.empty((N, 3, b - a), dtype=torch.float64)
+                                              for _ in range(2)) for a, b in self._ranges()]
 
     # ------------------------------------------------------------------------------------------
     def set_theta(self, theta):
@@ -125,13 +161,71 @@ class AdjointProblem:
             self._advance(step, mark - step)
             step = mark
             if mark % self.C == 0 and mark < self.nsteps:
-                self.checkpoints[mark] = (sysm.u.clone(), sysm.up.clone())
+                self._store_checkpoint(mark)
             if mark in self.obs_index:
                 loss += self._loss_term(sysm.u, mark)
                 if record:
                     self.obs_states.append(sysm.u.clone())
         self.u_final = sysm.u.clone()
+        if self.host_checkpoints:
+            for job in self._stage_jobs:
+                if job is not None:
+                    job.result()
         return loss
+
+    def _blocked(self):
+        return self.sample_block is not None or self.host_checkpoints
+
+    def _ranges(self):
+        S = self.sys.n_smp
+        B = S if self.sample_block is None else self.sample_block
+        return [(a, min(a + B, S)) for a in range(0, S, B)]
+
+    def _store_checkpoint(self, mark):
+        """Keep the pair (u_mark, u_mark-1): on the device, or per sample block (device copies of
+        the block's samples), or in host memory: device copies into one of two staging slots,
+        then the helper thread's copy to the host while the main thread goes on with the steps
+        (it waits only when a slot's previous copy is still running)."""
+        sysm = self.sys
+        if not self._blocked():
+            self.checkpoints[mark] = (sysm.u.clone(), sysm.up.clone())
+            return
+        ranges = self._ranges()
+        if not self.host_checkpoints:
+            self.checkpoints[mark] = [(sysm.u[..., a:b].contiguous(), sysm.up[..., a:b].contiguous())
+                                      for a, b in ranges]
+            return
+        main = torch.cuda.current_stream(sysm.device)
+        slot = self._stage_i
+        self._stage_i ^= 1
+        if self._stage_jobs[slot] is not None:
+            self._stage_jobs[slot].result()           # the slot's previous copy is out
+        if self._stage[slot] is None:
+            self._stage[slot] = [tuple(torch.empty((sysm.n_nodes, 3, b - a), dtype=torch.float64,
+                                                   device=sysm.device) for _ in range(2))
+                                 for a, b in ranges]
+        stage = self._stage[slot]
+        for (a, b), (su, sup) in zip(ranges, stage):
+            su.copy_(sysm.u[..., a:b])
+            sup.copy_(sysm.up[..., a:b])
+        ready = torch.cuda.Event()
+        ready.record(main)
+        host = self._host[mark]
+        self._stage_jobs[slot] = self._worker.submit(self._copy_pairs, ready, stage, host)
+        self.checkpoints[mark] = host
+
+    def _copy_pairs(self, ready, src, dst):
+        """Helper thread: once ``ready`` has passed on the device, copy the pairs ``src`` into
+        ``dst`` on the copy stream (pageable host copies return when done)."""
+        dev = self.sys.device
+        with torch.cuda.device(dev):
+            if ready is not None:
+                ready.synchronize()
+            with torch.cuda.stream(self._copy_stream):
+                for (su, sup), (du, dup) in zip(src, dst):
+                    du.copy_(su)
+                    dup.copy_(sup)
+            self._copy_stream.synchronize()
 
     # ------------------------------------------------------------------------------------------
     def _adjoint_call(self, mu1, mu2, out, obs_u=None, obs_ref=None, w=None, project=1):
@@ -177,16 +271,11 @@ class AdjointProblem:
         S = sysm.n_smp
         w = (torch.ones(S, dtype=torch.float64, device=dev) if grad_out is None
              else grad_out.to(dev, torch.float64).reshape(S).contiguous())
-        shape = sysm.state_shape()
-        zeros = torch.zeros(shape, dtype=torch.float64, device=dev)
-        mu = [torch.zeros(shape, dtype=torch.float64, device=dev) for _ in range(3)]
-        grad = torch.zeros((sysm.n_elms, S), dtype=torch.float64, device=dev)
+        if self._blocked():
+            return self._backward_blocks(w)
         N = self.nsteps
-        # mu_N = P g_N
-        if N in self.obs_index:
-            self._adjoint_call(zeros, zeros, mu[0], self.u_final, self._obs(N), w)
-        self._mu = [mu[0], zeros.clone(), mu[1]]              # mu_{n+1}, mu_{n+2}, spare
-        self._w, self._grad = w, grad
+        self._begin_reverse(w)
+        shape = sysm.state_shape()
         leaf = self.C if self.C2 is None else self.C2
         fused = self.fused()
         # states of the innermost segment (the pipelined sweep also holds the next one's, in the
@@ -215,16 +304,136 @@ class AdjointProblem:
                     for t0 in sorted(pairs, reverse=True):
                         self._reverse_leaf(t0, min(t0 + self.C2, s1), pairs[t0])
                 del pairs
+        return self._end_reverse()
+
+    def _begin_reverse(self, w):
+        """Adjoint state mu_{N+1} = 0, mu_N = P g_N and a zero gradient."""
+        sysm = self.sys
+        dev = sysm.device
+        shape = sysm.state_shape()
+        self._zeros = torch.zeros(shape, dtype=torch.float64, device=dev)
+        mu = [torch.zeros(shape, dtype=torch.float64, device=dev) for _ in range(2)]
+        self._grad = torch.zeros((sysm.n_elms, sysm.n_smp), dtype=torch.float64, device=dev)
+        N = self.nsteps
+        if N in self.obs_index:
+            self._adjoint_call(self._zeros, self._zeros, mu[0], self.u_final, self._obs(N), w)
+        self._mu = [mu[0], self._zeros.clone(), mu[1]]        # mu_{n+1}, mu_{n+2}, spare
+        self._w = w
+
+    def _end_reverse(self):
+        """The initial state's term (u_{-1} depends on K through u0 + beta du0); returns the
+        gradient and releases the sweep's buffers."""
+        sysm = self.sys
+        dev = sysm.device
+        S = sysm.n_smp
+        shape = sysm.state_shape()
+        grad, zeros = self._grad, self._zeros
         mu_n1 = self._mu[0]
         self._pool = self._tmp = None
+        self._mu = self._grad = self._zeros = None
         sysm.state = None
-        # initial state: u_{-1} depends on K through (u0 + beta du0)
         x0 = self.u0 + self.beta * self.du0
         if np.any(x0 != 0.0):
             lam_m1 = torch.empty(shape, dtype=torch.float64, device=dev)
             self._adjoint_call(zeros, mu_n1, lam_m1, project=0)
             x0_d = torch.as_tensor(np.repeat(x0.reshape(-1, 3)[:, :, None], S, axis=2)).to(dev)
             self._grad_call(lam_m1, x0_d, None, 0.5 * self.dt * self.dt, grad)
+        return grad
+
+    def _block_problems(self):
+        """One single-level problem per sample block, on the block's system (this system itself
+        when one block holds every sample)."""
+        if self._blocks is None:
+            self._blocks = []
+            for a, b in self._ranges():
+                sb = self.sys if (a, b) == (0, self.sys.n_smp) else self.sys.sample_block(a, b)
+                p = AdjointProblem(sb, self.load, self.nsteps, self.dt, beta=self.beta,
+                                   damp=self.damp, method=self.method, obs_steps=self.obs_steps,
+                                   checkpoint_every=self.C, u0=self.u0, du0=self.du0, fuse=False)
+                p.block_range = (a, b)
+                self._blocks.append(p)
+        return self._blocks
+
+    def _block_obs(self, k, j):
+        """Observation k restricted to sample block j (device; one host transfer per step k)."""
+        if self.u_obs is None:
+            return None
+        ranges = self._ranges()
+        if len(ranges) == 1:
+            return self._obs(k)
+        if k not in self._obs_blocks:
+            full = self._obs(k)
+            while len(self._obs_blocks) >= 2:                 # the steps of the current segment
+                del self._obs_blocks[next(iter(self._obs_blocks))]
+            self._obs_blocks[k] = [full[..., a:b].contiguous() for a, b in ranges]
+        return self._obs_blocks[k][j]
+
+    def _backward_blocks(self, w):
+        """Reverse sweep one sample block at a time: for each segment (last first), each block
+        recomputes its states from the block's checkpoint pair and runs its adjoint and gradient
+        steps through the segment.  Host checkpoint pairs come to the device one pair ahead on
+        the copy stream, in two alternating buffer slots."""
+        sysm = self.sys
+        dev = sysm.device
+        blocks = self._block_problems()
+        self._obs_blocks = {}
+        self._stage = [None, None]           # the forward pass's staging (its copies were waited on)
+        if blocks[0].sys is not sysm:
+            sysm.state = None                # the blocks recompute into their own buffers
+        for j, p in enumerate(blocks):
+            a, b = p.block_range
+            if p.sys is not sysm:
+                p.sys.refresh_block(sysm)
+            p.u_final = self.u_final if p.sys is sysm else self.u_final[..., a:b].contiguous()
+            p._obs = (lambda k, j=j: self._block_obs(k, j))
+            p._begin_reverse(w[a:b].contiguous())
+        big = blocks[0].sys.state_shape()
+        pool = [torch.empty(big, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]
+
+        def fit(buf, shape):                 # the first `shape` elements of a block-0 buffer
+            return buf.view(-1)[:int(np.prod(shape))].view(shape)
+
+        order = [(m, j) for m in sorted(self.checkpoints, reverse=True) for j in range(len(blocks))]
+        main = torch.cuda.current_stream(dev)
+        slots = [None, None]
+        free = [None, None]                  # main-stream events: a slot's last reader has run
+
+        def fetch(i):                        # copy pair i to a device slot on the helper thread
+            m, j = order[i]
+            k = i % 2
+            if slots[k] is None:
+                slots[k] = tuple(torch.empty(big, dtype=torch.float64, device=dev) for _ in range(2))
+            shape = blocks[j].sys.state_shape()
+            pair = tuple(fit(t, shape) for t in slots[k])
+            return self._worker.submit(self._copy_pairs, free[k], [self.checkpoints[m][j]],
+                                       [pair]), pair
+
+        pending = fetch(0) if self.host_checkpoints and order else None
+        for i, (m, j) in enumerate(order):
+            p = blocks[j]
+            if self.host_checkpoints:
+                job, pair = pending
+                job.result()
+                if i + 1 < len(order):
+                    pending = fetch(i + 1)
+            else:
+                pair = self.checkpoints[m][j]
+            shape = p.sys.state_shape()
+            p._pool = [fit(t, shape) for t in pool]
+            p._reverse_leaf(m, min(m + self.C, self.nsteps), pair)
+            if self.host_checkpoints:
+                ev = torch.cuda.Event()
+                ev.record(main)
+                free[i % 2] = ev
+        self._obs_blocks = {}
+        S = sysm.n_smp
+        grad = torch.empty((sysm.n_elms, S), dtype=torch.float64, device=dev)
+        for p in blocks:
+            a, b = p.block_range
+            g = p._end_reverse()
+            grad[:, a:b] = g
+        self.stats["fused_steps"] = 0
+        self.stats["fused"] = False
         return grad
 
     def _fusion_pays(self, pair, n):

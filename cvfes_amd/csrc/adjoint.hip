@@ -24,6 +24,7 @@ struct AdjArgs {
     double *out;           // mu_n (projected) or lambda_{-1} (unprojected)
     const double *lm;
     int64_t lm_sstride;
+    const double *minv;    // 1/lm (0 where lm = 0), same layout; NULL = divide by lm
     const uint8_t *fixed;
     const double *obs_u;   // u_n      (observation term, may be NULL)
     const double *obs_ref; // u_obs    (NULL = 0)
@@ -48,10 +49,28 @@ __device__ __forceinline__ Vec<VW> mass_at(const AdjArgs &a, int64_t node, int64
 }
 
 // Column term of the adjoint product at node col: x = dt^2/m_col (b1 mu1 - b0 mu2) (0 where the
-// mass is 0); K is symmetric, so the transposed product is the same gathered row product.
+// mass is 0); K is symmetric, so the transposed product is the same gathered row product.  With the
+// inverse masses (minv, 0 where m = 0) the scale is dt^2 * minv_col, one multiply per column
+// instead of an FP64 division per column and sample (the division sequence costs the kernel about
+// a sixth of its time at 256 samples).
 template <bool BETA, int VW>
 __device__ __forceinline__ void adj_column(const AdjArgs &a, int64_t col, int64_t s, double (&x)[3][VW]) {
-    const Vec<VW> mc = mass_at<VW>(a, col, s);
+    Vec<VW> q;
+    if (a.minv) {
+        if (a.lm_sstride) {
+            q = load_vec<VW>(a.minv + col * a.k.n_smp + s);
+#pragma unroll
+            for (int v = 0; v < VW; ++v) q.v[v] *= a.dt2;
+        } else {
+            const double qc = a.dt2 * a.minv[col];
+#pragma unroll
+            for (int v = 0; v < VW; ++v) q.v[v] = qc;
+        }
+    } else {
+        const Vec<VW> mc = mass_at<VW>(a, col, s);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) q.v[v] = mc.v[v] != 0.0 ? a.dt2 / mc.v[v] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const int64_t off = (col * 3 + j) * a.k.n_smp + s;
@@ -62,7 +81,7 @@ __device__ __forceinline__ void adj_column(const AdjArgs &a, int64_t col, int64_
         for (int v = 0; v < VW; ++v) {
             double comb = m1.v[v];
             if constexpr (BETA) comb = fma(a.bdt, m1.v[v] - m2.v[v], m1.v[v]);
-            x[j][v] = mc.v[v] != 0.0 ? comb * (a.dt2 / mc.v[v]) : 0.0;
+            x[j][v] = comb * q.v[v];
         }
     }
 }
@@ -157,30 +176,16 @@ struct GradArgs {
     const int32_t *elems, *cpos;
     const double *khat, *coef, *lm;
     int64_t lm_sstride;
+    const double *minv;    // 1/lm (0 where lm = 0), same layout; NULL = divide by lm
     const double *mu, *xc, *xp;
     double bdt, nu_scale;
     double *grad;
 };
 
-// sum_{a,b} nu_a^T khat_e(a,b) x_b for the thread's (element e, sample s), with nu = nu_scale * mu / m
-// and x = xc + bdt (xc - xp) (xp may be NULL).  `kh(ab)` gives the 9 entries of unit block ab.
+// sum_{a,b} nu_a^T khat_e(a,b) x_b over the element's 9 x 9 unit block; `kh(ab)` gives the 9
+// entries of unit block ab.
 template <class KH>
-__device__ __forceinline__ double grad_sum(const GradArgs &a, int64_t e, int64_t s, KH kh) {
-    int64_t nd[3];
-    double nu[9], x[9];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        nd[q] = a.elems[3 * e + q];
-        const double m = a.lm[a.lm_sstride ? nd[q] * a.n_smp + s : nd[q]];
-        const double sc = m != 0.0 ? a.nu_scale / m : 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int64_t off = (nd[q] * 3 + j) * a.n_smp + s;
-            nu[3 * q + j] = sc * a.mu[off];
-            const double c = a.xc[off];
-            x[3 * q + j] = a.xp ? fma(a.bdt, c - a.xp[off], c) : c;
-        }
-    }
+__device__ __forceinline__ double grad_form(const double (&nu)[9], const double (&x)[9], KH kh) {
     double acc = 0.0;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -199,21 +204,59 @@ __device__ __forceinline__ double grad_sum(const GradArgs &a, int64_t e, int64_t
     return acc;
 }
 
-// grad[e,s] += -coef[e,s] * grad_sum, one thread per (element, sample), unit blocks read from
-// memory.  UNI (sample count a multiple of 64): every wave holds one element, so its node ids and
-// unit blocks are scalar loads.  At most 80 registers (6 waves per SIMD or more): left alone, the
-// compiler hoists all 81 unit-block loads and runs at 3 waves per SIMD.
+// grad[e,s] = fma(-coef[e,s], sum_{a,b} nu_a^T khat_e(a,b) x_b, grad[e,s]) for the thread's
+// (element e, sample s), with nu = nu_scale * mu / m (nu_scale * minv with the inverse masses) and
+// x = xc + bdt (xc - xp) (x = xc when xp is NULL).
+template <class KH>
+__device__ __forceinline__ void grad_update(const GradArgs &a, int64_t e, int64_t s, KH kh) {
+    double nu[9], x[9];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int64_t nd = a.elems[3 * e + q];
+        const int64_t mi = a.lm_sstride ? nd * a.n_smp + s : nd;
+        double sc;
+        if (a.minv) {
+            sc = a.nu_scale * a.minv[mi];
+        } else {
+            const double m = a.lm[mi];
+            sc = m != 0.0 ? a.nu_scale / m : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int64_t off = (nd * 3 + j) * a.n_smp + s;
+            nu[3 * q + j] = sc * a.mu[off];
+            const double c = a.xc[off];
+            x[3 * q + j] = a.xp ? fma(a.bdt, c - a.xp[off], c) : c;
+        }
+    }
+    const int64_t idx = e * a.n_smp + s;
+    a.grad[idx] = fma(-a.coef[idx], grad_form(nu, x, kh), a.grad[idx]);
+}
+
+// One thread per (element, sample), unit blocks read from memory.  UNI (sample count a multiple
+// of 64): every wave holds one element and 64 of its samples, so its node ids and unit blocks are
+// scalar loads, and the waves run sample group first (all elements for samples 0-63, then
+// 64-127, ...), which keeps the gathered node data of one group in L2 between the elements that
+// share it.  At most 80 registers (6 waves per SIMD or more): left alone, the compiler hoists all
+// 81 unit-block loads and runs at 3 waves per SIMD.  (A form that applies two consecutive steps'
+// terms in one pass, reading u_{n-1}, coef and grad once, needs 123 registers and measured 10 %
+// slower per step.)
 template <bool UNI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad_accum(GradArgs a) {
     const int64_t g = xcd_block() * kBlock + threadIdx.x;
     if (g >= a.n_elms * a.n_smp) return;
-    int64_t e = g / a.n_smp;
-    if constexpr (UNI) e = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(e));
-    const int64_t s = g - e * a.n_smp;
-    const double acc = grad_sum(a, e, s, [&](int ab) {
+    int64_t e, s;
+    if constexpr (UNI) {
+        const int64_t w = g >> 6, group = w / a.n_elms;
+        e = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(w - group * a.n_elms));
+        s = group * 64 + (g & 63);
+    } else {
+        e = g / a.n_smp;
+        s = g - e * a.n_smp;
+    }
+    grad_update(a, e, s, [&](int ab) {
         return a.khat + 9 * static_cast<int64_t>(a.cpos[9 * e + ab]);
     });
-    a.grad[g] -= a.coef[g] * acc;
 }
 
 // The same with the unit blocks of the workgroup's elements staged in LDS first (cooperative
@@ -241,8 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_accum_lds(GradArgs a) {
     const int64_t e = g / a.n_smp;
     const int64_t s = g - e * a.n_smp;
     const double *ske = sk + (e - e0) * 81;
-    const double acc = grad_sum(a, e, s, [&](int ab) { return ske + 9 * ab; });
-    a.grad[g] -= a.coef[g] * acc;
+    grad_update(a, e, s, [&](int ab) { return ske + 9 * ab; });
 }
 
 template <int MF, bool UNI, int VW>
@@ -328,6 +370,7 @@ AdjArgs adjoint_args(const cvfes_system_t *sys, const double *mu1, const double 
     a.out = mu_out;
     a.lm = sys->lm;
     a.lm_sstride = sys->lm_sstride;
+    a.minv = sys->minv;
     a.fixed = sys->fixed;
     a.obs_u = obs_u;
     a.obs_ref = obs_ref;
@@ -341,6 +384,12 @@ AdjArgs adjoint_args(const cvfes_system_t *sys, const double *mu1, const double 
     return a;
 }
 
+// minv[i] = 1 / lm[i], 0 where lm[i] = 0.
+__global__ __launch_bounds__(kBlock) void k_inverse_mass(int64_t n, const double *__restrict__ lm,
+                                                         double *__restrict__ minv) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) minv[i] = lm[i] != 0.0 ? 1.0 / lm[i] : 0.0;
+}
 }  // namespace
 }  // namespace cvfes
 
@@ -419,7 +468,7 @@ extern "C" int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const i
                   "cvfes_grad_accum: null pointer");
     CVFES_REQUIRE(dt > 0.0, "cvfes_grad_accum: dt must be positive");
     const GradArgs a{sys->n_elms, sys->n_smp, elems, cpos, sys->khat, sys->coef, sys->lm,
-                     sys->lm_sstride, mu, x_cur, x_prev, beta / dt, nu_scale, grad};
+                     sys->lm_sstride, sys->minv, mu, x_cur, x_prev, beta / dt, nu_scale, grad};
     const dim3 grid(grid_for(sys->n_elms * sys->n_smp)), block(kBlock);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (sys->n_smp % 64 == 0)
@@ -429,4 +478,13 @@ extern "C" int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const i
     else
         hipLaunchKernelGGL(k_grad_accum<false>, grid, block, 0, st, a);
     return check_launch("k_grad_accum");
+}
+
+extern "C" int cvfes_inverse_mass(void *stream, int64_t n, const double *lm, double *minv) {
+    CVFES_REQUIRE(n >= 0, "cvfes_inverse_mass: negative size");
+    if (n == 0) return CVFES_OK;
+    CVFES_REQUIRE(lm && minv, "cvfes_inverse_mass: null pointer");
+    hipLaunchKernelGGL(k_inverse_mass, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       static_cast<hipStream_t>(stream), n, lm, minv);
+    return check_launch("k_inverse_mass");
 }

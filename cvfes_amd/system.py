@@ -8,6 +8,7 @@ the HIP kernels of ``libcvfes_mi355x.so``.
 """
 from __future__ import annotations
 
+import copy
 import ctypes
 
 import numpy as np
@@ -195,6 +196,9 @@ class MembraneSystem:
                                                   ptr(self.elems), ptr(self.area), ptr(self.normal),
                                                   ptr(self.vthick), self.thick_sstride, self.density,
                                                   MASS_SCALE[mass_form], ptr(self.lm), ptr(self.f1)))
+            # inverse masses of the adjoint and gradient kernels
+            self.minv = torch.empty_like(self.lm)
+            check(self.lib.cvfes_inverse_mass(st, self.lm.numel(), ptr(self.lm), ptr(self.minv)))
             # --- stored block-CSR values ---------------------------------------------------------
             self.values = None
             if store_values:
@@ -289,6 +293,44 @@ class MembraneSystem:
         self.row_records = _dev(rec, torch.int32, self.device)
         self.record_width = 2 + 2 * maxb
 
+    def sample_block(self, s0, s1):
+        """The system of samples ``s0 .. s1-1`` alone, for the matrix-free operator: it shares the
+        structure, element blocks, loads and per-node masses with this one and holds its own
+        copies of the block's coefficients (and per-sample thickness and masses), no stored
+        values and no state.  Samples do not interact, so a step of the block system computes
+        exactly (bit for bit) the block's samples of a step of this one.  :meth:`refresh_block`
+        copies the coefficients again after this system's change."""
+        s0, s1 = int(s0), int(s1)
+        if not 0 <= s0 < s1 <= self.n_smp:
+            raise ValueError("sample block must satisfy 0 <= s0 < s1 <= n_smp")
+        b = copy.copy(self)
+        b.n_smp = s1 - s0
+        b.block_of = (s0, s1)
+        b.coef = self.coef[:, s0:s1].contiguous()
+        b.ave_E = None if self.ave_E is None else self.ave_E[:, s0:s1].contiguous()
+        if self.thick_sstride:
+            b.vthick = self.vthick[:, s0:s1].contiguous()
+            b.lm = self.lm[:, s0:s1].contiguous()
+            b.minv = None if self.minv is None else self.minv[:, s0:s1].contiguous()
+        b.values = b.uvalues = None
+        b.row_records, b.record_width = None, 0
+        b.state, b.update, b.ug = None, None, None
+        b._sys = b._make_struct()
+        return b
+
+    def refresh_block(self, parent):
+        """Copy this sample block's coefficients (and per-sample thickness and masses) from the
+        system it was taken from (:meth:`sample_block`)."""
+        s0, s1 = self.block_of
+        self.coef.copy_(parent.coef[:, s0:s1])
+        if parent.ave_E is not None and self.ave_E is not None:
+            self.ave_E.copy_(parent.ave_E[:, s0:s1])
+        if parent.thick_sstride:
+            self.vthick.copy_(parent.vthick[:, s0:s1])
+            self.lm.copy_(parent.lm[:, s0:s1])
+            if self.minv is not None:
+                self.minv.copy_(parent.minv[:, s0:s1])
+
     def release_values(self):
         """Drop the stored values (matrix-free runs only need coef and khat)."""
         self.values = None
@@ -309,7 +351,7 @@ class MembraneSystem:
             color_elems=ptr(self.color_elems), elems=ptr(self.elems), cpos=ptr(self.cpos),
             n_colors=self.n_colors, max_color_elems=self.max_color_elems,
             waves_per_simd=self.waves_per_simd, row_records=ptr(self.row_records),
-            record_width=self.record_width)
+            record_width=self.record_width, minv=ptr(self.minv))
 
     def set_variant(self, variant):
         """Select the step kernel variant (names in ``_lib.VARIANTS``); the symmetric ones form the

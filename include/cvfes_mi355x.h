@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CVFES_ABI_VERSION 11
+#define CVFES_ABI_VERSION 12
 
 #define CVFES_OK 0
 #define CVFES_ERR_ARG (-1)   /* invalid argument (null pointer, bad size)        */
@@ -102,6 +102,10 @@ typedef struct cvfes_system {
      * match; the results are the same either way. */
     const int32_t *row_records;
     int64_t record_width;
+    /* Inverse lumped masses 1/lm (0 where lm = 0), same layout as lm (may be NULL).  The adjoint
+     * and gradient kernels scale by them (dt^2 * minv, nu_scale * minv) instead of dividing by lm
+     * per column and sample; with NULL they divide.  The forward step kernels do not read it. */
+    const double *minv;
 } cvfes_system_t;
 
 /* Step kernel variants (cvfes_system_t.variant). */
@@ -201,6 +205,13 @@ int cvfes_lumped_mass_load(void *stream, int64_t n_nodes, int64_t n_smp, const i
                            const int32_t *nent, const int32_t *elems, const double *area,
                            const double *normal, const double *vthick, int64_t thick_sstride,
                            double density, double mass_scale, double *lm, double *f1);
+
+/*
+ * minv[i] = 1 / lm[i] (0 where lm[i] = 0) for i < n: cvfes_system_t.minv of the adjoint and
+ * gradient kernels (n = N, or N*S when the masses are per sample).
+ * No counterpart in the reference (the adjoint is an addition of this library).
+ */
+int cvfes_inverse_mass(void *stream, int64_t n, const double *lm, double *minv);
 
 /*
  * Symmetric stored values: uvalues[q][ij][s] = values[ublk[q]][ij][s] for the nnzu upper blocks
@@ -313,13 +324,14 @@ int cvfes_adjoint_fused_step(void *stream, const cvfes_system_t *sys, int method
 
 /*
  * Gradient with respect to theta = log E per element and sample (no counterpart in the reference):
- *   grad[e,s] -= coef[e,s] * sum_ab (nu_scale mu/lm)_a^T khat_e(a,b) x_b,
+ *   grad[e,s] = fma(-coef[e,s], sum_ab (nu_scale mu/lm)_a^T khat_e(a,b) x_b, grad[e,s]),
  *   x = x_cur + (beta/dt)(x_cur - x_prev)  (x_prev may be NULL: x = x_cur).
  * elems [E][3], cpos [9E] (block slot of element e's (a,b) in khat), grad [E][S] accumulated.
  */
 int cvfes_grad_accum(void *stream, const cvfes_system_t *sys, const int32_t *elems,
                      const int32_t *cpos, const double *mu, const double *x_cur,
                      const double *x_prev, double dt, double beta, double nu_scale, double *grad);
+
 
 /*
  * Reference geometry of a mesh for the kernels that work on the moved geometry X + u.

@@ -94,7 +94,7 @@ def test_adjoint_matches_torch_autograd(require_gpu, method, beta, damp, C, u0, 
 
 
 @pytest.mark.parametrize("method,S", [("stored", 8), ("matfree", 8), ("stored", 5), ("matfree", 66),
-                                      ("stored-sym", 8), ("stored-sym", 5)])
+                                      ("stored-sym", 8), ("stored-sym", 5), ("matfree", 128)])
 def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
     """The pipelined sweep (recomputation of a segment fused with the adjoint steps of the next)
     gives the gradient of the separate kernels bit for bit, with beta, damping, an initial
@@ -126,6 +126,80 @@ def test_fused_reverse_sweep_is_bit_identical(require_gpu, method, S):
     assert np.max(np.abs(grads[0])) > 0
     for g in grads[1:]:
         assert np.array_equal(grads[0], g)
+
+
+@pytest.mark.parametrize("method,S", [("matfree", 128), ("stored-sym", 128), ("matfree", 64)])
+def test_node_mass_scales_match_per_sample_masses(require_gpu, method, S):
+    """The adjoint and gradient kernels scale by the inverse masses, read per node or (with the
+    same masses stored per sample) per sample: loss and gradient are the same bit for bit, for
+    separate and fused sweeps.  Without the inverse masses (the kernels divide by the masses)
+    they agree to rounding."""
+    from cvfes_amd.adjoint import AdjointProblem
+    method, storage = method_storage(method)
+    nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 7, storage=storage)
+    nsteps, dt = 40, 4e-6
+    p = sine(nsteps, dt)
+    obs = [8, 25, 40]
+    rng = np.random.default_rng(4)
+    u_obs = torch.as_tensor(1e-5 * rng.standard_normal((len(obs), nodes.shape[0], 3, S))).cuda()
+    u0 = 1e-6 * rng.standard_normal(3 * nodes.shape[0])
+    out = []
+    for per_sample in (False, True):
+        if per_sample:
+            sysm.lm = sysm.lm[:, None].expand(sysm.n_nodes, S).contiguous()
+            sysm.minv = sysm.minv[:, None].expand(sysm.n_nodes, S).contiguous()
+            sysm.vthick = sysm.vthick[:, None].expand(sysm.n_nodes, S).contiguous()
+            sysm.thick_sstride = 1
+            sysm._sys = sysm._make_struct()
+        for fuse in (False, True):
+            prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0,
+                                  method=method, obs_steps=obs, u_obs=u_obs, checkpoint_every=9,
+                                  u0=u0, fuse=fuse)
+            prob.set_theta(torch.as_tensor(theta).cuda())
+            loss = prob.forward().cpu().numpy()
+            out.append((loss, prob.backward().cpu().numpy()))
+    assert np.max(np.abs(out[0][1])) > 0
+    for loss, g in out[1:]:
+        assert np.array_equal(out[0][0], loss)
+        assert np.array_equal(out[0][1], g)
+    sysm.minv = None
+    sysm._sys = sysm._make_struct()
+    prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0, method=method,
+                          obs_steps=obs, u_obs=u_obs, checkpoint_every=9, u0=u0, fuse=False)
+    prob.set_theta(torch.as_tensor(theta).cuda())
+    assert np.array_equal(out[0][0], prob.forward().cpu().numpy())
+    g = prob.backward().cpu().numpy()
+    assert np.max(np.abs(g - out[0][1])) <= 1e-13 * np.max(np.abs(out[0][1]))
+
+
+@pytest.mark.parametrize("S,B", [(10, 4), (128, 64)])
+def test_sample_blocks_and_host_checkpoints_are_bit_identical(require_gpu, S, B):
+    """The reverse sweep by sample blocks (blocks of B samples, the last one shorter when B does
+    not divide S), with the checkpoints on the device or in pinned host memory, and with host
+    checkpoints for the whole ensemble, gives the loss and gradient of the plain one-level sweep
+    bit for bit (observations on steps inside segments and on the last step, an initial
+    displacement, beta and damping)."""
+    from cvfes_amd.adjoint import AdjointProblem
+    nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 5, store=False)
+    nsteps, dt = 50, 4e-6
+    p = sine(nsteps, dt)
+    obs = [7, 20, 21, 44, 50]
+    rng = np.random.default_rng(6)
+    u_obs = torch.as_tensor(1e-5 * rng.standard_normal((len(obs), nodes.shape[0], 3, S))).pin_memory()
+    u0 = 1e-6 * rng.standard_normal(3 * nodes.shape[0])
+    wts = torch.as_tensor(np.resize([1.0, 0.5, 2.0, -1.0], S)).cuda()
+    out = []
+    for blk, host in ((None, False), (B, False), (B, True), (None, True)):
+        prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0,
+                              method="matfree", obs_steps=obs, u_obs=u_obs, checkpoint_every=9,
+                              u0=u0, fuse=False, sample_block=blk, host_checkpoints=host)
+        prob.set_theta(torch.as_tensor(theta).cuda())
+        loss = prob.forward().cpu().numpy()
+        out.append((loss, prob.backward(wts).cpu().numpy()))
+    assert np.max(np.abs(out[0][1])) > 0
+    for loss, g in out[1:]:
+        assert np.array_equal(out[0][0], loss)
+        assert np.array_equal(out[0][1], g)
 
 
 def test_gradient_independent_of_checkpoint_interval(require_gpu):

@@ -698,3 +698,4 @@ def test_batch_refuses_steps_past_load_tables():
     assert launched == [0, 1, 2, 0] and ens.step_index == 4
     ens.run(2, graph=False)
     assert ens.step_index == 6
+
