@@ -199,10 +199,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all OpenMP threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matfree", action="store_true")
-    ap.add_argument("--adjoint-block", type=int, default=0,
-                    help="combined workload: reverse sweep by sample blocks of this size (0 = two levels)")
+    ap.add_argument("--adjoint-block", type=int, default=64,
+                    help="combined workload: reverse sweep by sample blocks of this size, one checkpoint "
+                         "level (0 = the whole ensemble, two levels)")
     ap.add_argument("--adjoint-every", type=int, default=0, help="combined workload: checkpoint interval")
     ap.add_argument("--adjoint-host", action="store_true", help="combined workload: checkpoints in host memory")
+    ap.add_argument("--adjoint-sequential", action="store_true",
+                    help="combined workload: gradient steps on the main stream (default: beside the "
+                         "adjoint steps)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward ratio of the differentiation workload")
     args = ap.parse_args()
@@ -737,8 +741,8 @@ def run_combined(args):
         u_obs[i].copy_(sysm.u)
     log(f"observations ({time.perf_counter() - t_start:.0f} s)")
     if args.adjoint_block:
-        # one checkpoint level, the reverse sweep by sample blocks so that a block's recomputed
-        # segment fits the device beside the checkpoints
+        # one checkpoint level (pairs every 125 steps: 101 GB), the reverse sweep by blocks of 64
+        # samples so that a block's recomputed segment (124 states of 0.79 GB) fits beside them
         C, C2, B = args.adjoint_every or 125, None, args.adjoint_block
     else:
         # two levels (pairs every 200 steps, and every 20 inside a segment): the whole ensemble's
@@ -746,7 +750,7 @@ def run_combined(args):
         C, C2, B = args.adjoint_every or 200, 20, None
     prob = AdjointProblem(sysm, load, nsteps, dt, beta=beta, method="matfree", obs_steps=obs,
                           u_obs=u_obs, checkpoint_every=C, inner_every=C2, sample_block=B,
-                          host_checkpoints=args.adjoint_host)
+                          host_checkpoints=args.adjoint_host, concurrent=not args.adjoint_sequential)
     prob.set_theta(torch.as_tensor(theta).cuda())
     torch.cuda.reset_peak_memory_stats()
     torch.cuda.synchronize()
@@ -771,6 +775,7 @@ def run_combined(args):
                    "n_samples": S, "dt": dt, "beta": beta, "observations": len(obs),
                    "checkpoint_every": C, "inner_every": C2, "sample_block": B,
                    "checkpoints_in": "host memory" if args.adjoint_host else "device memory",
+                   "concurrent_gradient": not args.adjoint_sequential,
                    "adjoint_operator": "matfree"},
         "step_operators": per_step, "stored_vs_matfree_rel_max_err": agree,
         "forward_s": fwd, "adjoint_s": adj, "adjoint_over_forward": adj / fwd,

@@ -30,8 +30,8 @@ reduction and the autograd plumbing.
 """
 from __future__ import annotations
 
-import concurrent.futures
 import ctypes
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -67,11 +67,16 @@ class AdjointProblem:
         sweep bit for bit.
     host_checkpoints : keep the checkpoints in (pageable) host memory, one pair per segment and
         sample block, copied by a helper thread while the kernels run.
+    concurrent : run each gradient step on a stream of its own beside the adjoint steps (the two
+        kernels are independent and neither fills the memory system alone) in the sweeps that
+        recompute a segment before reversing it (not the pipelined one).  Same result bit for
+        bit.
     """
 
     def __init__(self, system: MembraneSystem, load: LoadTables, nsteps, dt, beta=0.0, damp=0.0,
                  method="stored", obs_steps=(), u_obs=None, checkpoint_every=100, u0=None, du0=None,
-                 inner_every=None, fuse="auto", sample_block=None, host_checkpoints=False):
+                 inner_every=None, fuse="auto", sample_block=None, host_checkpoints=False,
+                 concurrent=False):
         self.sys = system
         self.load = load
         self.nsteps = int(nsteps)
@@ -91,6 +96,8 @@ class AdjointProblem:
         self.du0 = np.zeros(3 * N) if du0 is None else np.asarray(du0, np.float64)
         self.checkpoints = {}
         self.stats = {}
+        self.concurrent = bool(concurrent)
+        self._grad_stream = torch.cuda.Stream(device=system.device) if self.concurrent else None
         S = system.n_smp
         self.sample_block = None if sample_block is None else int(sample_block)
         self.host_checkpoints = bool(host_checkpoints)
@@ -110,7 +117,7 @@ class AdjointProblem:
             self._stage_i = 0
             self._host = {}
             if self.host_checkpoints:
-                self._worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+                self._worker = ThreadPoolExecutor(max_workers=1)
                 for mark in range(0, self.nsteps, self.C):
                     self._host[mark] = [tuple(torch.empty((N, 3, b - a), dtype=torch.float64)
                                               for _ in range(2)) for a, b in self._ranges()]
@@ -347,9 +354,14 @@ class AdjointProblem:
             self._blocks = []
             for a, b in self._ranges():
                 sb = self.sys if (a, b) == (0, self.sys.n_smp) else self.sys.sample_block(a, b)
+                if sb is not self.sys and self.sys.waves_per_simd:
+                    # the occupancy cap that suits the whole ensemble need not suit a block
+                    sb.autotune_occupancy(self.load, self.dt, self.beta, self.damp,
+                                          method=self.method, steps=5)
                 p = AdjointProblem(sb, self.load, self.nsteps, self.dt, beta=self.beta,
                                    damp=self.damp, method=self.method, obs_steps=self.obs_steps,
-                                   checkpoint_every=self.C, u0=self.u0, du0=self.du0, fuse=False)
+                                   checkpoint_every=self.C, u0=self.u0, du0=self.du0, fuse=False,
+                                   concurrent=self.concurrent)
                 p.block_range = (a, b)
                 self._blocks.append(p)
         return self._blocks
@@ -561,17 +573,33 @@ class AdjointProblem:
             self._advance(n, 1)
             states[n + 1] = self._pool[i]
         mu_n1, mu_n2, spare = self._mu
+        gs = self._grad_stream
+        main = torch.cuda.current_stream(sysm.device) if gs is not None else None
+        done = {}
         for n in range(s1 - 1, s0 - 1, -1):
             # gradient of step n: mu_{n+1}, x_n = u_n + (beta/dt)(u_n - u_{n-1})
-            self._grad_call(mu_n1, states[n], states[n - 1] if self.beta != 0.0 else None,
-                            self.dt * self.dt, self._grad)
+            xp = states[n - 1] if self.beta != 0.0 else None
+            if gs is None:
+                self._grad_call(mu_n1, states[n], xp, self.dt * self.dt, self._grad)
+            else:
+                # on the gradient stream, once mu_{n+1} is out (main-stream order); the gradient
+                # steps keep their order on that stream, so the sum is the same bit for bit
+                gs.wait_stream(main)
+                with torch.cuda.stream(gs):
+                    self._grad_call(mu_n1, states[n], xp, self.dt * self.dt, self._grad)
+                    done[n] = torch.cuda.Event()
+                    done[n].record(gs)
             if n == 0:
                 break
             obs_u = states[n] if n in self.obs_index else None
+            if n + 2 in done:              # `spare` holds mu_{n+3}, read by step n+2's gradient
+                main.wait_event(done.pop(n + 2))
             self._adjoint_call(mu_n1, mu_n2, spare, obs_u,
                                self._obs(n) if obs_u is not None else None,
                                self._w if obs_u is not None else None)
             mu_n2, mu_n1, spare = mu_n1, spare, mu_n2
+        if gs is not None:
+            main.wait_stream(gs)           # the segment's states are overwritten next
         self._mu = [mu_n1, mu_n2, spare]
 
 

@@ -299,11 +299,11 @@ void adj_launch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
     }
 }
 
-// Lane width and row mapping as the generic forward kernel chooses them (two samples per lane
-// for even counts; wave-uniform rows when they keep 3/4 of the lanes busy).
+// Lane width and row mapping as the generic forward kernel chooses them (generic_vw; wave-uniform
+// rows when they keep 3/4 of the lanes busy).
 template <int MF>
 void adj_dispatch(hipStream_t st, const AdjArgs &a, bool beta, bool damp) {
-    if (a.k.n_smp % 2 == 0) {
+    if (generic_vw(a.k.n_smp) == 2) {
         if (use_uniform<2>(a.k.n_smp)) adj_launch<MF, true, 2>(st, a, beta, damp);
         else adj_launch<MF, false, 2>(st, a, beta, damp);
     } else {
@@ -326,7 +326,7 @@ void fused_launch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta
 
 template <int MF>
 void fused_dispatch(hipStream_t st, const StepArgs &f, const AdjArgs &a, bool beta, bool damp) {
-    if (f.k.n_smp % 2 == 0) {
+    if (generic_vw(f.k.n_smp) == 2) {
         if (use_uniform<2>(f.k.n_smp)) fused_launch<MF, true, 2>(st, f, a, beta, damp);
         else fused_launch<MF, false, 2>(st, f, a, beta, damp);
     } else {

@@ -384,7 +384,7 @@ void launch_spmv_u(hipStream_t st, const StepArgs &a) {
 
 template <int OP>
 void spmv_op(hipStream_t st, const StepArgs &a) {
-    if (a.k.n_smp % 2 == 0) launch_spmv_u<OP, 2>(st, a);
+    if (generic_vw(a.k.n_smp) == 2) launch_spmv_u<OP, 2>(st, a);
     else launch_spmv_u<OP, 1>(st, a);
 }
 
@@ -430,7 +430,7 @@ void step_u(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool dam
 
 template <int OP>
 void step_generic(hipStream_t st, const StepArgs &a, int64_t step, bool beta, bool damp) {
-    if (a.k.n_smp % 2 == 0) step_u<OP, 2>(st, a, step, beta, damp);
+    if (generic_vw(a.k.n_smp) == 2) step_u<OP, 2>(st, a, step, beta, damp);
     else step_u<OP, 1>(st, a, step, beta, damp);
 }
 

@@ -38,6 +38,15 @@ inline bool use_uniform(int64_t n_smp) {
     return 4 * l >= 3 * padded;
 }
 
+// Samples per lane of the generic row kernels (step, product, adjoint, fused): two for even
+// counts, except where only one keeps the rows wave-uniform (64 samples: 32 two-sample lanes
+// would put two rows in a wave, and the element blocks would come in per lane instead of through
+// scalar loads).  The per-sample arithmetic is the same either way.
+inline int generic_vw(int64_t n_smp) {
+    if (n_smp % 2 != 0) return 1;
+    return (!use_uniform<2>(n_smp) && use_uniform<1>(n_smp)) ? 1 : 2;
+}
+
 // Operator forms of the row functions (template argument OP): stored values in full storage,
 // matrix-free (element coefficients x unit element blocks), stored values in symmetric storage.
 constexpr int kOpStored = 0;
