@@ -137,13 +137,17 @@ def test_combined_two_level_checkpoints_bit_identical(combined):
     u_obs = 1e-5 * torch.randn((len(obs), N, 3, S), dtype=torch.float64, device="cuda", generator=g)
     theta = torch.as_tensor(np.log(E)).cuda()
     grads = []
-    for C, C2 in ((8, None), (20, 4)):
+    # one level, two levels, and the combined workload's schedule (one level by blocks of 64
+    # samples with the gradient steps on their own stream)
+    for C, C2, B, conc in ((8, None, None, False), (20, 4, None, False), (8, None, 64, True)):
         prob = AdjointProblem(sysm, load, nsteps, dt, beta=case.beta, method="matfree",
-                              obs_steps=obs, u_obs=u_obs, checkpoint_every=C, inner_every=C2)
+                              obs_steps=obs, u_obs=u_obs, checkpoint_every=C, inner_every=C2,
+                              sample_block=B, concurrent=conc)
         prob.set_theta(theta)
         prob.forward()
         grads.append(prob.backward().cpu().numpy())
         del prob
         torch.cuda.empty_cache()
     assert np.all(np.isfinite(grads[0])) and np.max(np.abs(grads[0])) > 0
-    assert np.array_equal(grads[1], grads[0])
+    for g_other in grads[1:]:
+        assert np.array_equal(g_other, grads[0])
