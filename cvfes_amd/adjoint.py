@@ -42,6 +42,57 @@ from .system import LoadTables, MembraneSystem
 __all__ = ["AdjointProblem", "EnsembleLoss"]
 
 
+class _ObsStream:
+    """The observations restricted to samples a..b-1 on the device, in the order a reverse sweep
+    takes them (last step first): the problem's helper thread copies them into a few buffers in
+    rotation, each one once the kernel that read the observation it held has been issued."""
+
+    def __init__(self, prob, a, b, depth=3):
+        self.p, self.a, self.b = prob, a, b
+        self.order = sorted(prob.obs_steps, reverse=True)
+        self.pos = {k: i for i, k in enumerate(self.order)}
+        n = min(depth, len(self.order))
+        shape = (prob.sys.n_nodes, 3, b - a)
+        self.ring = [torch.empty(shape, dtype=torch.float64, device=prob.sys.device) for _ in range(n)]
+        self.free = [None] * n
+        self.jobs = {}
+        self.next = 0
+        self.last = None
+        for _ in range(n):
+            self._issue()
+
+    def _issue(self):
+        i = self.next
+        if i >= len(self.order):
+            return
+        self.next += 1
+        k = self.order[i]
+        slot = i % len(self.ring)
+        src = self.p.u_obs[self.p.obs_index[k]][..., self.a:self.b]
+        self.jobs[i] = self.p._worker.submit(self.p._copy_in, self.free[slot], self.ring[slot], src)
+
+    def get(self, k):
+        i = self.pos[k]
+        if self.last is not None:
+            if i != self.last + 1:
+                raise RuntimeError("observations must be taken last step first, each once")
+            # the kernel that read the previous observation has been issued: its slot is free
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(self.p.sys.device))
+            self.free[self.last % len(self.ring)] = ev
+            self._issue()
+        elif i != 0:
+            raise RuntimeError("observations must be taken last step first, each once")
+        self.jobs.pop(i).result()
+        self.last = i
+        return self.ring[i % len(self.ring)]
+
+    def close(self):
+        for job in self.jobs.values():
+            job.result()
+        self.jobs = {}
+
+
 class AdjointProblem:
     """Forward + adjoint of ``nsteps`` steps on a :class:`MembraneSystem`.
 
@@ -109,15 +160,16 @@ class AdjointProblem:
                 raise ValueError("sample blocks and host checkpoints run one checkpoint level")
             if method != "matfree" and self._ranges() != [(0, S)]:
                 raise ValueError("sample blocks run the matrix-free operator")
-            self.fuse = False
+            if self.host_checkpoints:
+                self.fuse = False
             dev = system.device
             self._copy_stream = torch.cuda.Stream(device=dev)
             self._stage = [None, None]          # device staging of the checkpoint copies
             self._stage_jobs = [None, None]     # helper-thread copies out of each staging slot
             self._stage_i = 0
             self._host = {}
+            self._worker = ThreadPoolExecutor(max_workers=1)
             if self.host_checkpoints:
-                self._worker = ThreadPoolExecutor(max_workers=1)
                 for mark in range(0, self.nsteps, self.C):
                     self._host[mark] = [tuple(torch.empty((N, 3, b - a), dtype=torch.float64)
                                               for _ in range(2)) for a, b in self._ranges()]
@@ -220,6 +272,16 @@ class AdjointProblem:
         host = self._host[mark]
         self._stage_jobs[slot] = self._worker.submit(self._copy_pairs, ready, stage, host)
         self.checkpoints[mark] = host
+
+    def _copy_in(self, ready, dst, src):
+        """Helper thread: once ``ready`` has passed on the device, copy ``src`` into the device
+        buffer ``dst`` on the copy stream."""
+        with torch.cuda.device(self.sys.device):
+            if ready is not None:
+                ready.synchronize()
+            with torch.cuda.stream(self._copy_stream):
+                dst.copy_(src)
+            self._copy_stream.synchronize()
 
     def _copy_pairs(self, ready, src, dst):
         """Helper thread: once ``ready`` has passed on the device, copy the pairs ``src`` into
@@ -397,13 +459,18 @@ class AdjointProblem:
             if p.sys is not sysm:
                 p.sys.refresh_block(sysm)
             p.u_final = self.u_final if p.sys is sysm else self.u_final[..., a:b].contiguous()
-            p._obs = (lambda k, j=j: self._block_obs(k, j))
-            p._begin_reverse(w[a:b].contiguous())
         big = blocks[0].sys.state_shape()
-        pool = [torch.empty(big, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]
 
         def fit(buf, shape):                 # the first `shape` elements of a block-0 buffer
             return buf.view(-1)[:int(np.prod(shape))].view(shape)
+
+        if not self.host_checkpoints:
+            return self._backward_block_major(w, blocks, big, fit)
+        for j, p in enumerate(blocks):
+            a, b = p.block_range
+            p._obs = (lambda k, j=j: self._block_obs(k, j))
+            p._begin_reverse(w[a:b].contiguous())
+        pool = [torch.empty(big, dtype=torch.float64, device=dev) for _ in range(self.C - 1)]
 
         order = [(m, j) for m in sorted(self.checkpoints, reverse=True) for j in range(len(blocks))]
         main = torch.cuda.current_stream(dev)
@@ -446,6 +513,46 @@ class AdjointProblem:
             grad[:, a:b] = g
         self.stats["fused_steps"] = 0
         self.stats["fused"] = False
+        return grad
+
+    def _backward_block_major(self, w, blocks, big, fit):
+        """Device checkpoints: the whole reverse sweep of one sample block after the other, so
+        that a block can run the pipelined sweep (each segment recomputed during the reverse
+        steps of the one after it, one fused kernel per pair of steps) where that is faster.  The
+        block's observations come to the device one after the other, a few ahead, on the helper
+        thread."""
+        sysm = self.sys
+        dev = sysm.device
+        S = sysm.n_smp
+        grad = torch.empty((sysm.n_elms, S), dtype=torch.float64, device=dev)
+        marks = sorted(self.checkpoints)
+        fused = bool(self.fuse) and len(marks) >= 2 and blocks[0]._generic_forward_kernel()
+        pool = [torch.empty(big, dtype=torch.float64, device=dev)
+                for _ in range(self.C if fused else self.C - 1)]
+        self.stats["fused_steps"] = 0
+        for j, p in enumerate(blocks):
+            a, b = p.block_range
+            obs = _ObsStream(self, a, b) if self.u_obs is not None else None
+            p._obs = (lambda k, obs=obs: None if obs is None else obs.get(k))
+            p._begin_reverse(w[a:b].contiguous())
+            shape = p.sys.state_shape()
+            p._pool = [fit(t, shape) for t in pool]
+            pairs = {m: self.checkpoints[m][j] for m in marks}
+            if fused and self.fuse == "auto" and j == 0:
+                fused = p._fusion_pays(pairs[marks[-1]], marks[-1])
+                self.stats["fusion_timing_ms"] = p.stats.get("fusion_timing_ms")
+            if fused:
+                p.stats["fused_steps"] = 0
+                p._reverse_pipelined(pairs, self.C, self.nsteps)
+                self.stats["fused_steps"] += p.stats["fused_steps"]
+            else:
+                p._pool = p._pool[:self.C - 1]
+                for m in reversed(marks):
+                    p._reverse_leaf(m, min(m + self.C, self.nsteps), pairs[m])
+            grad[:, a:b] = p._end_reverse()
+            if obs is not None:
+                obs.close()
+        self.stats["fused"] = fused
         return grad
 
     def _fusion_pays(self, pair, n):

@@ -176,9 +176,10 @@ def test_node_mass_scales_match_per_sample_masses(require_gpu, method, S):
 def test_sample_blocks_and_host_checkpoints_are_bit_identical(require_gpu, S, B):
     """The reverse sweep by sample blocks (blocks of B samples, the last one shorter when B does
     not divide S), with the checkpoints on the device or in host memory, with host checkpoints
-    for the whole ensemble, and with the gradient steps on a stream beside the adjoint steps,
-    gives the loss and gradient of the plain one-level sweep bit for bit (observations on steps
-    inside segments and on the last step, an initial displacement, beta and damping)."""
+    for the whole ensemble, with the gradient steps on a stream beside the adjoint steps, and
+    block by block with the pipelined (fused) sweep, gives the loss and gradient of the plain
+    one-level sweep bit for bit (observations on steps inside segments and on the last step, an
+    initial displacement, beta and damping)."""
     from cvfes_amd.adjoint import AdjointProblem
     nodes, el, bdy, theta, sysm = build(20, 9, 0.3, 1.2, S, 5, store=False)
     nsteps, dt = 50, 4e-6
@@ -189,15 +190,18 @@ def test_sample_blocks_and_host_checkpoints_are_bit_identical(require_gpu, S, B)
     u0 = 1e-6 * rng.standard_normal(3 * nodes.shape[0])
     wts = torch.as_tensor(np.resize([1.0, 0.5, 2.0, -1.0], S)).cuda()
     out = []
-    for blk, host, conc in ((None, False, False), (B, False, False), (B, True, False),
-                            (None, True, False), (None, False, True), (B, False, True)):
+    for blk, host, conc, fuse in ((None, False, False, False), (B, False, False, False),
+                                  (B, True, False, False), (None, True, False, False),
+                                  (None, False, True, False), (B, False, True, False),
+                                  (B, False, False, True)):
         prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0,
                               method="matfree", obs_steps=obs, u_obs=u_obs, checkpoint_every=9,
-                              u0=u0, fuse=False, sample_block=blk, host_checkpoints=host,
+                              u0=u0, fuse=fuse, sample_block=blk, host_checkpoints=host,
                               concurrent=conc)
         prob.set_theta(torch.as_tensor(theta).cuda())
         loss = prob.forward().cpu().numpy()
         out.append((loss, prob.backward(wts).cpu().numpy()))
+        assert (prob.stats["fused_steps"] > 0) == fuse
     assert np.max(np.abs(out[0][1])) > 0
     for loss, g in out[1:]:
         assert np.array_equal(out[0][0], loss)
