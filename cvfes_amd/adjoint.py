@@ -113,9 +113,11 @@ class AdjointProblem:
         on the problem's own states first and keeps the faster form (the fused kernel carries
         two columns of states and loses where they overflow L2); the gradient is the same bit
         for bit either way.
-    sample_block : run the reverse sweep for blocks of this many samples at a time (matrix-free;
-        one checkpoint level; no fused kernels).  The gradient is that of the whole-ensemble
-        sweep bit for bit.
+    sample_block : run the reverse sweep for blocks of this many samples at a time (matrix-free,
+        one checkpoint level): with device checkpoints one block after the other, each with the
+        pipelined sweep when ``fuse`` allows it; with host checkpoints segment by segment over
+        the blocks, without fused kernels.  The gradient is that of the whole-ensemble sweep bit
+        for bit.
     host_checkpoints : keep the checkpoints in (pageable) host memory, one pair per segment and
         sample block, copied by a helper thread while the kernels run.
     concurrent : run each gradient step on a stream of its own beside the adjoint steps (the two
