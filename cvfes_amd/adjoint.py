@@ -54,7 +54,11 @@ class _ObsStream:
         n = min(depth, len(self.order))
         shape = (prob.sys.n_nodes, 3, b - a)
         self.ring = [torch.empty(shape, dtype=torch.float64, device=prob.sys.device) for _ in range(n)]
-        self.free = [None] * n
+        # fresh buffers may reuse memory that kernels queued on the main stream still read or
+        # write (the allocator orders reuse on that stream only): the first copies wait for them
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(prob.sys.device))
+        self.free = [ev] * n
         self.jobs = {}
         self.next = 0
         self.last = None
@@ -484,6 +488,9 @@ class AdjointProblem:
             k = i % 2
             if slots[k] is None:
                 slots[k] = tuple(torch.empty(big, dtype=torch.float64, device=dev) for _ in range(2))
+                # the helper writes them on its own stream: after what the main stream has queued
+                free[k] = torch.cuda.Event()
+                free[k].record(main)
             shape = blocks[j].sys.state_shape()
             pair = tuple(fit(t, shape) for t in slots[k])
             return self._worker.submit(self._copy_pairs, free[k], [self.checkpoints[m][j]],

@@ -202,6 +202,16 @@ def test_sample_blocks_and_host_checkpoints_are_bit_identical(require_gpu, S, B)
         loss = prob.forward().cpu().numpy()
         out.append((loss, prob.backward(wts).cpu().numpy()))
         assert (prob.stats["fused_steps"] > 0) == fuse
+    # observations on the device: the blocks' copies of them are made on the helper stream into
+    # freshly allocated buffers, which must wait for the work queued on the main stream
+    u_dev = u_obs.cuda()
+    for fuse in (False, True):
+        prob = AdjointProblem(sysm, sysm.loads(p), nsteps, dt, beta=1e-7, damp=15.0,
+                              method="matfree", obs_steps=obs, u_obs=u_dev, checkpoint_every=9,
+                              u0=u0, fuse=fuse, sample_block=B)
+        prob.set_theta(torch.as_tensor(theta).cuda())
+        loss = prob.forward().cpu().numpy()
+        out.append((loss, prob.backward(wts).cpu().numpy()))
     assert np.max(np.abs(out[0][1])) > 0
     for loss, g in out[1:]:
         assert np.array_equal(out[0][0], loss)
