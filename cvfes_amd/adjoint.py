@@ -42,6 +42,18 @@ from .system import LoadTables, MembraneSystem
 __all__ = ["AdjointProblem", "EnsembleLoss"]
 
 
+_HELPER = None
+
+
+def _helper():
+    """The helper thread (one for the process) that copies checkpoints and observations while
+    the main thread keeps queuing kernels."""
+    global _HELPER
+    if _HELPER is None:
+        _HELPER = ThreadPoolExecutor(max_workers=1, thread_name_prefix="cvfes-copy")
+    return _HELPER
+
+
 class _ObsStream:
     """The observations restricted to samples a..b-1 on the device, in the order a reverse sweep
     takes them (last step first): the problem's helper thread copies them into a few buffers in
@@ -174,7 +186,7 @@ class AdjointProblem:
             self._stage_jobs = [None, None]     # helper-thread copies out of each staging slot
             self._stage_i = 0
             self._host = {}
-            self._worker = ThreadPoolExecutor(max_workers=1)
+            self._worker = _helper()
             if self.host_checkpoints:
                 for mark in range(0, self.nsteps, self.C):
                     self._host[mark] = [tuple(torch.empty((N, 3, b - a), dtype=torch.float64)
