@@ -248,7 +248,7 @@ def main():
     total = args.warmup + args.steps
     load = sysm.loads(case.pressures(0, max(total, args.cpu_steps + 1), dt))
     if args.waves_per_simd == "auto":
-        occ = sysm.autotune_occupancy(load, dt, beta, method=args.method)
+        occ = sysm.autotune_occupancy(load, dt, beta, method=args.method, steps=20)
     else:
         sysm.set_occupancy(int(args.waves_per_simd))
         occ = {"waves_per_simd": int(args.waves_per_simd)}
@@ -316,7 +316,7 @@ def main():
                 sysm.set_storage(storage)
             sysm.set_variant(variant)
             sysm.set_row_order(aorder)
-            aocc = sysm.autotune_occupancy(load, dt, beta, method=other, steps=5)
+            aocc = sysm.autotune_occupancy(load, dt, beta, method=other, steps=20)
             msteps = max(20, args.steps // 2)
             mwall, mk = time_steps(sysm, load, dt, beta, other, args.warmup, msteps)
             mbytes = algorithmic_bytes(sysm, other)
