@@ -49,7 +49,7 @@ class BatchedEnsemble:
         # rows are wave-uniform with two (packed rows take one)
         self.vw_hint = 0
         self.cap = False          # cap the kernel at 64 registers (8 waves per SIMD, possible spills)
-        self.waves_per_simd = 0   # cap on the waves per SIMD (3..7, unused LDS; 0 = none)
+        self.waves_per_simd = 0   # cap on the waves per SIMD (2..7, unused LDS; 0 = none)
         self.kinds = 0
         self.phase = 0
         self.step_index = 0

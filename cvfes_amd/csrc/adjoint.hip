@@ -408,8 +408,8 @@ int check_adjoint_args(const cvfes_system_t *sys, int method, const double *mu1,
     CVFES_REQUIRE(mu1 && mu2 && mu_out && sys->lm, "%s: null pointer", who);
     CVFES_REQUIRE(mu_out != mu1 && mu_out != mu2, "%s: output aliases an input", who);
     CVFES_REQUIRE(dt > 0.0, "%s: dt must be positive", who);
-    CVFES_REQUIRE(sys->waves_per_simd == 0 || (sys->waves_per_simd >= 3 && sys->waves_per_simd <= 7),
-                  "%s: waves_per_simd must be 0 or 3..7", who);
+    CVFES_REQUIRE(sys->waves_per_simd == 0 || (sys->waves_per_simd >= 2 && sys->waves_per_simd <= 7),
+                  "%s: waves_per_simd must be 0 or 2..7", who);
     return CVFES_OK;
 }
 

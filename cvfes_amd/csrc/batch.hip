@@ -199,10 +199,10 @@ BatchKernel batch_kernel(int64_t kinds, bool one) {
 }
 
 // kinds from cvfes_batch_plan (bits 0..4, at least one lane path), the caller's register cap (bit
-// 5) and waves-per-SIMD cap (bits 6..8: 0 or 3..7).
+// 5) and waves-per-SIMD cap (bits 6..8: 0 or 2..7).
 bool valid_kinds(int64_t kinds) {
     const int64_t w = (kinds >> 6) & 7;
-    return kinds >= 1 && kinds < 512 && (kinds & 19) && (w == 0 || w >= 3);
+    return kinds >= 1 && kinds < 512 && (kinds & 19) && (w == 0 || w >= 2);
 }
 
 __global__ void k_counter_add(int64_t *counter, int64_t by) {

@@ -181,18 +181,23 @@ __global__ __launch_bounds__(kBlock) void k_step_rowload(StepArgs a, int64_t ste
 // Stored step on the symmetric storage: the row's blocks are read from the upper entries
 // (uvalues[vblk[p]]), transposed for the lower ones (col < row).  Every upper block is read twice per
 // step, once by its own row and once, transposed, by its column's row, 1 and ntheta rows later in
-// the XCD-ordered sweep, so the second read is served by the caches and the HBM value stream is
-// about halved.  Value loads: NT = 0 plain, 1 all non-temporal, 2 non-temporal for the mirrored
-// (second) reads only.  Loads are issued for the whole row before the arithmetic, as in the
-// row-load kernel.
+// the XCD-ordered sweep (or one strip width later in strip order), so the second read is served by
+// L2 and the HBM value stream is about halved.  All loads are buffer loads (one per-lane offset
+// register for the whole row, resources in scalar registers): 124 VGPRs, four waves per SIMD.
+// Value loads: NT = 0 plain (the fastest), 1 all non-temporal, 2 non-temporal for the mirrored
+// (second) reads only.  NT = 2 has two load paths per block, which the compiler hoists into one
+// wave per SIMD holding the whole row's loads.
 // REC: the row's structure comes from its record (`rec`, see cvfes_system_t.row_records): one
-// load chain shorter than row order -> indptr -> indices/vblk, which matters at one wave per SIMD.
+// load chain shorter than row order -> indptr -> indices/vblk.
+// Resources: states 3*S*8 bytes per node, values 9*S*8 bytes per block (resolve_variant keeps
+// 72*S below 2^31).
 template <bool BETA, bool DAMP, int NT, int MAXB, int VW, bool REC>
 __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *__restrict__ uvalues,
                                                      const int32_t *__restrict__ vblk,
                                                      const int32_t *__restrict__ rec, int64_t step) {
     typedef typename VecT<VW>::type V;
     constexpr int RECW = 2 + 2 * MAXB;
+    constexpr int AUX_FIRST = NT == 1 ? kAuxNT : 0;
     int32_t row;
     int64_t s;
     bool active;
@@ -208,12 +213,17 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
     const int64_t S = k.n_smp;
     const int32_t p0 = REC ? 0 : k.indptr[row];
     const int32_t nb = REC ? r[1] : k.indptr[row + 1] - p0;
+    const uint32_t lo = static_cast<uint32_t>(s) * 8u;   // the lane's byte offset in every row
+    const uint32_t sb = static_cast<uint32_t>(S) * 8u;   // bytes of one entry across the samples
     V uo[3], po[3];
+    {
+        const auto ru = buffer_rsrc(a.u + static_cast<int64_t>(row) * 3 * S, 3 * sb);
+        const auto rp = buffer_rsrc(a.up + static_cast<int64_t>(row) * 3 * S, 3 * sb);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const int64_t off = (static_cast<int64_t>(row) * 3 + i) * S + s;
-        uo[i] = ldv<VW, false>(a.u + off);
-        po[i] = ldv<VW, false>(a.up + off);
+        for (int i = 0; i < 3; ++i) {
+            uo[i] = ldb<VW, 0>(ru, lo, i * sb);
+            po[i] = ldb<VW, 0>(rp, lo, i * sb);
+        }
     }
     V acc[3] = {splat<VW>(0.0), splat<VW>(0.0), splat<VW>(0.0)};
     if (nb > 0) {
@@ -224,26 +234,28 @@ __global__ __launch_bounds__(kBlock) void k_step_sym(StepArgs a, const double *_
             const int32_t col = REC ? r[2 + b] : k.indices[pb];
             const bool tr = col < row;
             V w[3];
+            const auto cu = buffer_rsrc(a.u + static_cast<int64_t>(col) * 3 * S, 3 * sb);
+            const auto cp = buffer_rsrc(a.up + static_cast<int64_t>(col) * 3 * S, 3 * sb);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int64_t off = (static_cast<int64_t>(col) * 3 + j) * S + s;
-                const V xu = ldv<VW, false>(a.u + off);
+                const V xu = ldb<VW, 0>(cu, lo, j * sb);
                 if constexpr (BETA) {
-                    const V xp = ldv<VW, false>(a.up + off);
+                    const V xp = ldb<VW, 0>(cp, lo, j * sb);
                     w[j] = xu + a.bdt * (xu - xp);
                 } else {
                     w[j] = xu;
                 }
             }
-            const double *base = uvalues + static_cast<int64_t>(REC ? r[2 + MAXB + b] : vblk[pb]) * 9 * S + s;
+            const int64_t ub = REC ? r[2 + MAXB + b] : vblk[pb];
+            const auto rv = buffer_rsrc(uvalues + ub * 9 * S, 9 * sb);
             V kv[9];
             if (NT == 2 && tr) {
 #pragma unroll
-                for (int ij = 0; ij < 9; ++ij) kv[ij] = ldv<VW, true>(base + (3 * (ij % 3) + ij / 3) * S);
+                for (int ij = 0; ij < 9; ++ij) kv[ij] = ldb<VW, kAuxNT>(rv, lo, (3 * (ij % 3) + ij / 3) * sb);
             } else {
 #pragma unroll
                 for (int ij = 0; ij < 9; ++ij)
-                    kv[ij] = ldv<VW, NT == 1>(base + (tr ? 3 * (ij % 3) + ij / 3 : ij) * S);
+                    kv[ij] = ldb<VW, AUX_FIRST>(rv, lo, (tr ? 3 * (ij % 3) + ij / 3 : ij) * sb);
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -355,8 +367,8 @@ int validate(const cvfes_system_t *sys, int method, const char *who) {
     else
         CVFES_REQUIRE(sys->cptr && sys->celem && sys->khat && sys->coef,
                       "%s: matrix-free method needs cptr/celem/khat/coef", who);
-    CVFES_REQUIRE(sys->waves_per_simd == 0 || (sys->waves_per_simd >= 3 && sys->waves_per_simd <= 7),
-                  "%s: waves_per_simd must be 0 or 3..7", who);
+    CVFES_REQUIRE(sys->waves_per_simd == 0 || (sys->waves_per_simd >= 2 && sys->waves_per_simd <= 7),
+                  "%s: waves_per_simd must be 0 or 2..7", who);
     return CVFES_OK;
 }
 
@@ -671,11 +683,13 @@ int64_t resolve_variant(const cvfes_system_t *sys, bool mf) {
     // one, which reads either storage.
     const bool sym_only = !mf && !sys->values && sys->uvalues && sys->vblk;
     // Measured on MI355X (102,400-node tube, 128 samples): the stored operator is fastest with the
-    // symmetric kernel whose mirrored reads are non-temporal, or, in full storage, with the
-    // row-load kernel and non-temporal value loads; the matrix-free operator with the generic
-    // kernel (the row-load form needs too many registers for enough waves in flight).
+    // symmetric kernel and plain loads (the mirrored reads must stay in L2), or, in full storage,
+    // with the row-load kernel and non-temporal value loads; the matrix-free operator with the
+    // generic kernel (the row-load form needs too many registers for enough waves in flight).
+    // The symmetric kernel's buffer resources hold 72*S bytes (one block across the samples).
+    const bool sym_ok = maxb && sys->uvalues && sys->vblk && 72 * S <= INT32_MAX;
     const int64_t fallback = mf ? CVFES_VARIANT_GENERIC
-                                : (sym_only ? (maxb ? CVFES_VARIANT_SYM_NTL : CVFES_VARIANT_GENERIC)
+                                : (sym_only ? (sym_ok ? CVFES_VARIANT_SYM : CVFES_VARIANT_GENERIC)
                                             : CVFES_VARIANT_ROWLOAD_NT);
     if (v == CVFES_VARIANT_AUTO) v = fallback;
     if (sym_only && v != CVFES_VARIANT_SYM && v != CVFES_VARIANT_SYM_NT && v != CVFES_VARIANT_SYM_NTL)
@@ -695,7 +709,7 @@ int64_t resolve_variant(const cvfes_system_t *sys, bool mf) {
         v = fallback;
     }
     if (v == CVFES_VARIANT_SYM || v == CVFES_VARIANT_SYM_NT || v == CVFES_VARIANT_SYM_NTL) {
-        if (!mf && maxb && sys->uvalues && sys->vblk) return v;
+        if (!mf && sym_ok) return v;
         v = fallback;
     }
     if (maxb && v != CVFES_VARIANT_GENERIC) {

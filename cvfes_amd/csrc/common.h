@@ -60,10 +60,10 @@ __host__ __device__ inline bool stream_out(int64_t n_nodes, int64_t n_smp) {
 }
 
 // Unused LDS per 256-thread workgroup (one wave on each SIMD) that caps a kernel at w waves per
-// SIMD: a CU's 160 KiB then holds w such workgroups and no more.  0 (no cap) unless 3 <= w <= 7.
+// SIMD: a CU's 160 KiB then holds w such workgroups and no more.  0 (no cap) unless 2 <= w <= 7.
 inline unsigned occupancy_lds(int64_t w) {
     constexpr int64_t kLdsPerCu = 160 * 1024;
-    if (w < 3 || w > 7) return 0;
+    if (w < 2 || w > 7) return 0;
     return static_cast<unsigned>(kLdsPerCu / (w + 1) + 1024);   // in (kLdsPerCu/(w+1), kLdsPerCu/w]
 }
 

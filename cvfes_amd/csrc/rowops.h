@@ -331,6 +331,31 @@ __device__ __forceinline__ typename VecT<VW>::type ldv(const double *p) {
         return *reinterpret_cast<const V *>(p);
 }
 
+// Buffer loads: a resource (base address and size in bytes, built in scalar registers from
+// wave-uniform values) and a 32-bit per-lane byte offset.  Every load of a row then shares one
+// per-lane address register (the sample offset) instead of holding a 64-bit address per load,
+// which is what lets the symmetric step kernel keep four waves per SIMD.  Word 3 of the resource
+// is the gfx9 raw-access form (32-bit data format, no swizzle).  AUX is the load's cache policy:
+// 0 plain, kAuxNT non-temporal.  Every offset the kernels form lies inside its resource, so the
+// hardware's range check (which would return zeros) never applies.
+constexpr int kAuxNT = 2;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const double *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), static_cast<short>(0),
+                                             static_cast<int>(bytes), 0x00020000);
+}
+
+template <int VW, int AUX>
+__device__ __forceinline__ typename VecT<VW>::type ldb(__amdgpu_buffer_rsrc_t r, uint32_t lane_off,
+                                                       uint32_t off) {
+    if constexpr (VW == 2)
+        return __builtin_bit_cast(dbl2, __builtin_amdgcn_raw_buffer_load_b128(
+                                            r, static_cast<int>(lane_off), static_cast<int>(off), AUX));
+    else
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                              r, static_cast<int>(lane_off), static_cast<int>(off), AUX));
+}
+
 template <int VW>
 __device__ __forceinline__ typename VecT<VW>::type splat(double x) {
     if constexpr (VW == 2) return dbl2{x, x};

@@ -412,11 +412,11 @@ class MembraneSystem:
         self._sys = self._make_struct()
 
     def set_occupancy(self, waves_per_simd=0):
-        """Cap the step kernels that stage nothing in LDS at ``waves_per_simd`` (3..7; 0 = no cap)
+        """Cap the step kernels that stage nothing in LDS at ``waves_per_simd`` (2..7; 0 = no cap)
         wavefronts per SIMD.  The results do not depend on it."""
         w = int(waves_per_simd)
-        if w != 0 and not 3 <= w <= 7:
-            raise ValueError("waves_per_simd must be 0 or 3..7")
+        if w != 0 and not 2 <= w <= 7:
+            raise ValueError("waves_per_simd must be 0 or 2..7")
         self.waves_per_simd = w
         self._sys = self._make_struct()
 
@@ -450,7 +450,7 @@ class MembraneSystem:
         return best
 
     def autotune_occupancy(self, load, dt, beta=0.0, damp=0.0, method="stored", steps=10,
-                           candidates=(0, 6, 5, 4)):
+                           candidates=(0, 6, 5, 4, 3, 2)):
         """Time ``steps`` steps under each occupancy cap in ``candidates`` (waves per SIMD, 0 = no
         cap) and keep the fastest; returns ``{"waves_per_simd", "ms"}``.  Large rows (many
         samples, long rings) run faster with fewer rows in flight per XCD; the displacements do not

@@ -89,7 +89,7 @@ typedef struct cvfes_system {
     int64_t max_color_elems;    /* largest colour (launch bound)                                  */
     /* Cap on the wavefronts per SIMD of the step, adjoint and product kernels that stage nothing
      * in LDS (generic, row-load, symmetric), enforced with unused LDS per workgroup: 0 = as many
-     * as the registers allow, else 3..7.  Fewer rows in flight keep a larger share of their
+     * as the registers allow, else 2..7.  Fewer rows in flight keep a larger share of their
      * gathered neighbour data in the XCD's L2 when the rows are large (many samples, long rings);
      * the results do not depend on it. */
     int64_t waves_per_simd;
@@ -434,7 +434,7 @@ int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void 
  * (k_local alone when counter is NULL).  n_waves: persistent grid size in wavefronts (0 = 16 per
  * compute unit).  kinds: the value cvfes_batch_plan returned, plus the caller's launch choices:
  * bit 5 caps the kernel at 64 registers (8 waves per SIMD, spilling where the paths need more),
- * bits 6..8 hold a cap on the waves per SIMD (3..7, through unused LDS per workgroup; 0 = none). */
+ * bits 6..8 hold a cap on the waves per SIMD (2..7, through unused LDS per workgroup; 0 = none). */
 int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks, int phase,
                      const int64_t *counter, int64_t k_local, int64_t n_waves, int64_t kinds);
 

@@ -282,9 +282,12 @@ def test_step_kernel_resolution_without_gpu():
     assert kern(128, "ebe", "matfree") == V["generic"]          # no element colouring
     assert kern(128, "ebe", "matfree", colors=True) == V["ebe"]
     # symmetric storage only (no full values): the symmetric kernel, or the generic one
-    for v in ("auto", "rowload_nt", "rowload1", "sym_ntl", "lds"):
-        assert kern(128, v, "stored", uvalues=True, full=False) == V["sym_ntl"], v
-    assert kern(128, "sym", "stored", uvalues=True, full=False) == V["sym"]
+    for v in ("auto", "rowload_nt", "rowload1", "sym", "lds"):
+        assert kern(128, v, "stored", uvalues=True, full=False) == V["sym"], v
+    assert kern(128, "sym_ntl", "stored", uvalues=True, full=False) == V["sym_ntl"]
+    # a block across the samples must fit the symmetric kernel's 31-bit buffer resources
+    assert kern(1 << 25, "auto", "stored", uvalues=True, full=False) == V["generic"]
+    assert kern(1 << 24, "auto", "stored", uvalues=True, full=False) == V["sym"]
     assert kern(128, "generic", "stored", uvalues=True, full=False) == V["generic"]
     assert kern(100, "auto", "stored", uvalues=True, full=False) == V["generic"]
     assert kern(128, "auto", "stored", uvalues=True, full=False, limits=(7, 3, 6)) == V["generic"]
