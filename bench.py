@@ -188,8 +188,8 @@ def main():
     ap.add_argument("--storage", default="sym", choices=["sym", "full"],
                     help="stored values: upper blocks only (sym) or every block (full)")
     ap.add_argument("--variant", default="auto", help="step kernel variant (cvfes_amd._lib.VARIANTS)")
-    ap.add_argument("--waves-per-simd", default="auto", choices=["auto", "0", "3", "4", "5", "6", "7"],
-                    help="occupancy cap of the timed step (auto: the fastest of none, 6, 5, 4)")
+    ap.add_argument("--waves-per-simd", default="auto", choices=["auto", "0", "2", "3", "4", "5", "6", "7"],
+                    help="occupancy cap of the timed step (auto: the fastest of none, 6, 5, 4, 3, 2)")
     ap.add_argument("--row-order", default="auto", choices=["auto", "natural", "strips"],
                     help="row visiting order of the timed step (auto: the fastest of natural and "
                          "strips 32/64 on 20 steps each; the result does not depend on it)")
