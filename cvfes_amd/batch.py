@@ -50,6 +50,9 @@ class BatchedEnsemble:
         self.vw_hint = 0
         self.cap = False          # cap the kernel at 64 registers (8 waves per SIMD, possible spills)
         self.waves_per_simd = 0   # cap on the waves per SIMD (2..7, unused LDS; 0 = none)
+        # rows wider than a wavefront: visit them sample group first (all rows for the first 64
+        # lanes' samples, then the next group) instead of row by row
+        self.groups_first = False
         self.kinds = 0
         self.phase = 0
         self.step_index = 0
@@ -97,7 +100,8 @@ class BatchedEnsemble:
         self._drop_graphs()
 
     def _kinds(self):
-        return self.kinds | (32 if self.cap else 0) | (self.waves_per_simd << 6)
+        return (self.kinds | (32 if self.cap else 0) | (self.waves_per_simd << 6)
+                | (512 if self.groups_first else 0))
 
     def _drop_graphs(self):
         for g in self._graphs.values():
@@ -172,11 +176,12 @@ class BatchedEnsemble:
     def autotune(self, candidates=None, steps=6, lane_widths=(0, 1, 2), caps=(False, True),
                  occupancies=(0, 6, 5, 4)):
         """Pick the lane width, register cap and grid size (wavefronts) with the shortest step,
-        then the cap on the waves per SIMD for that choice."""
+        then the cap on the waves per SIMD for that choice, then the wave order of rows wider
+        than a wavefront (row by row or sample group first)."""
         n_grid = len(candidates) if candidates is not None else 5
         n_occ = sum(1 for o in occupancies if o != 0)
         # every timed candidate runs 1 + steps eager steps of the ensemble's load tables
-        self._check_steps((len(lane_widths) * len(caps) * n_grid + n_occ) * (steps + 1))
+        self._check_steps((len(lane_widths) * len(caps) * n_grid + n_occ + 1) * (steps + 1))
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         best = None
         saved = (self.phase, self.step_index)
@@ -207,12 +212,20 @@ class BatchedEnsemble:
             if dt < best[3]:
                 best = best[:3] + (dt, occ)
         self.waves_per_simd = best[4]
+        if any(int(m.lanes) > 64 and not int(m.packed) for m in self.host_meshes):
+            self.groups_first = True
+            dt = self._time(steps)
+            if dt < best[3]:
+                best = best[:3] + (dt, best[4])
+            else:
+                self.groups_first = False
         # the tuning steps advanced the state: restart the bookkeeping from the current roles
         self._sync_roles()
         self._replan()
         names = {0: "two where even", 1: "one everywhere", 2: "two where rows are uniform"}
         self.tuned = {"lane_width": names[best[0]], "register_cap": best[1], "n_waves": best[2],
-                      "waves_per_simd": best[4], "ms_per_step": best[3] * 1e3, "from_step": saved[1]}
+                      "waves_per_simd": best[4], "sample_groups_first": self.groups_first,
+                      "ms_per_step": best[3] * 1e3, "from_step": saved[1]}
         return self.tuned
 
     def _time(self, steps):

@@ -103,7 +103,8 @@ __device__ __forceinline__ void batch_dispatch(const cvfes_batch_mesh_t &d, int 
 template <bool ONE, bool U2, bool P2, bool V1, bool STORED, bool BD>
 __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh,
                                            int64_t n_tasks, int phase,
-                                           const int64_t *__restrict__ counter, int64_t k_local) {
+                                           const int64_t *__restrict__ counter, int64_t k_local,
+                                           int groups_first) {
     constexpr int64_t kWavesPerBlock = kBlock / 64;
     // The grid is a multiple of 8 blocks and block b runs on XCD b mod 8: the XCD's waves walk the
     // XCD's item list (its slice of every mesh, mesh after mesh) in contiguous ranges.
@@ -149,8 +150,20 @@ __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict_
             if (P2 && (!V1 || vw == 2)) batch_dispatch<2, STORED, BD>(d, phase, n, row, s, active);
             else if (V1) batch_dispatch<1, STORED, BD>(d, phase, n, row, s, active);
         } else {
-            const int32_t row = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(g / lanes));
-            int64_t s = static_cast<int64_t>(g - static_cast<uint32_t>(row) * lanes) * vw;
+            int32_t row;
+            int64_t s;
+            if (groups_first && lanes > 64) {
+                // sample group first: wave `local` takes group local / n_nodes (64*vw samples) of
+                // row local % n_nodes, so the rows in flight carry half (or less) of their samples
+                const uint32_t nn = static_cast<uint32_t>(d.sys.n_nodes);
+                const uint32_t grp = static_cast<uint32_t>(local) / nn;
+                row = __builtin_amdgcn_readfirstlane(
+                    static_cast<int32_t>(static_cast<uint32_t>(local) - grp * nn));
+                s = static_cast<int64_t>(grp * 64u + static_cast<uint32_t>(lane)) * vw;
+            } else {
+                row = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(g / lanes));
+                s = static_cast<int64_t>(g - static_cast<uint32_t>(row) * lanes) * vw;
+            }
             const bool active = s < S;
             if (!active) s = S - vw;
             if (U2 && (!V1 || vw == 2)) batch_dispatch<2, STORED, BD>(d, phase, n, row, s, active);
@@ -164,11 +177,13 @@ __device__ __forceinline__ void batch_body(const cvfes_batch_mesh_t *__restrict_
 template <bool ONE, bool U2, bool P2, bool V1, bool STORED, bool BD, bool CAP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP ? 8 : 1)))
 void k_batch_step(const cvfes_batch_mesh_t *__restrict__ meshes, int n_mesh, int64_t n_tasks,
-                  int phase, const int64_t *__restrict__ counter, int64_t k_local) {
-    batch_body<ONE, U2, P2, V1, STORED, BD>(meshes, n_mesh, n_tasks, phase, counter, k_local);
+                  int phase, const int64_t *__restrict__ counter, int64_t k_local, int groups_first) {
+    batch_body<ONE, U2, P2, V1, STORED, BD>(meshes, n_mesh, n_tasks, phase, counter, k_local,
+                                            groups_first);
 }
 
-using BatchKernel = void (*)(const cvfes_batch_mesh_t *, int, int64_t, int, const int64_t *, int64_t);
+using BatchKernel = void (*)(const cvfes_batch_mesh_t *, int, int64_t, int, const int64_t *, int64_t,
+                             int);
 
 // Smallest instance whose lane paths cover the plan's: (uniform two-sample, packed two-sample,
 // one-sample) in {(1,1,1), (1,1,0), (1,0,1), (1,0,0), (0,0,1)}.
@@ -199,10 +214,10 @@ BatchKernel batch_kernel(int64_t kinds, bool one) {
 }
 
 // kinds from cvfes_batch_plan (bits 0..4, at least one lane path), the caller's register cap (bit
-// 5) and waves-per-SIMD cap (bits 6..8: 0 or 2..7).
+// 5), waves-per-SIMD cap (bits 6..8: 0 or 2..7) and sample-group-first wave order (bit 9).
 bool valid_kinds(int64_t kinds) {
     const int64_t w = (kinds >> 6) & 7;
-    return kinds >= 1 && kinds < 512 && (kinds & 19) && (w == 0 || w >= 2);
+    return kinds >= 1 && kinds < 1024 && (kinds & 19) && (w == 0 || w >= 2);
 }
 
 __global__ void k_counter_add(int64_t *counter, int64_t by) {
@@ -308,7 +323,7 @@ extern "C" int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh
     hipLaunchKernelGGL(batch_kernel(kinds & 63, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock),
                        occupancy_lds((kinds >> 6) & 7), static_cast<hipStream_t>(stream),
                        static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks, phase,
-                       counter, k_local);
+                       counter, k_local, static_cast<int>((kinds >> 9) & 1));
     return check_launch("k_batch_step");
 }
 
@@ -340,7 +355,8 @@ extern "C" int cvfes_batch_graph_create(const void *dev_meshes, int n_mesh, int6
         hipLaunchKernelGGL(batch_kernel(kinds & 63, w >= n_tasks), dim3(waves_to_blocks(w)), dim3(kBlock),
                            occupancy_lds((kinds >> 6) & 7), cs,
                            static_cast<const cvfes_batch_mesh_t *>(dev_meshes), n_mesh, n_tasks,
-                           static_cast<int>((phase0 + k) % 3), counter, k);
+                           static_cast<int>((phase0 + k) % 3), counter, k,
+                           static_cast<int>((kinds >> 9) & 1));
     hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, cs, counter, block_steps);
     hipGraph_t g = nullptr;
     err = hipStreamEndCapture(cs, &g);

@@ -434,7 +434,10 @@ int cvfes_batch_plan(void *stream, cvfes_batch_mesh_t *meshes, int n_mesh, void 
  * (k_local alone when counter is NULL).  n_waves: persistent grid size in wavefronts (0 = 16 per
  * compute unit).  kinds: the value cvfes_batch_plan returned, plus the caller's launch choices:
  * bit 5 caps the kernel at 64 registers (8 waves per SIMD, spilling where the paths need more),
- * bits 6..8 hold a cap on the waves per SIMD (2..7, through unused LDS per workgroup; 0 = none). */
+ * bits 6..8 hold a cap on the waves per SIMD (2..7, through unused LDS per workgroup; 0 = none),
+ * bit 9 orders the waves of a mesh whose rows take more than one wavefront sample group first
+ * (every row for the first 64 lanes' samples, then the next group) instead of row by row; the
+ * results do not depend on it. */
 int cvfes_batch_step(void *stream, const void *dev_meshes, int n_mesh, int64_t n_tasks, int phase,
                      const int64_t *counter, int64_t k_local, int64_t n_waves, int64_t kinds);
 

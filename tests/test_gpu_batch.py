@@ -26,11 +26,19 @@ SPECS = [  # ntheta, nz, R, S, method, dt, beta, damp
 ]
 
 
-def make_ensemble(nsteps, seed=0, storage="full"):
+# rows wider than one wavefront: 256 and 200 samples at two per lane (128 lanes, uniform rows)
+WIDE = [
+    (24, 10, 0.3, 256, "matfree", 3e-6, 1e-7, 0.0),
+    (20, 12, 0.4, 200, "stored", 4e-6, 0.0, 10.0),
+    (16, 8, 0.3, 16, "matfree", 5e-6, 0.0, 0.0),
+]
+
+
+def make_ensemble(nsteps, seed=0, storage="full", specs=SPECS):
     from cvfes_amd.system import MembraneSystem
     rng = np.random.default_rng(seed)
     systems, loads = [], []
-    for (nt, nz, R, S, method, dt, beta, damp) in SPECS:
+    for (nt, nz, R, S, method, dt, beta, damp) in specs:
         nodes, el, bdy = tube_geometry(nt, nz, R, 10 * R)
         E = 7e6 * np.exp(0.1 * rng.standard_normal((el.shape[0], S)))
         s = MembraneSystem(nodes, el, bdy, S, vthickness=np.full(nodes.shape[0], 0.07), elmE=E,
@@ -71,6 +79,31 @@ def test_batched_equals_per_mesh(require_gpu, vw_hint, cap, waves, storage):
     for a, b in zip(batched, single):
         assert np.all(np.isfinite(a)) and np.max(np.abs(a)) > 0
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("storage", ["full", "sym"])
+def test_sample_group_order_equals_per_mesh(require_gpu, storage):
+    """Rows two wavefronts wide (256 and 200 samples) visited sample group first (every row for
+    the first 128 samples, then the rest) give the per-mesh result bit for bit, eagerly and in
+    graph replays."""
+    from cvfes_amd.batch import BatchedEnsemble
+    nsteps = 36
+    systems, loads = make_ensemble(nsteps, seed=7, storage=storage, specs=WIDE)
+    cols = list(zip(*WIDE))
+    ens = BatchedEnsemble(systems, loads, dts=list(cols[5]), methods=list(cols[4]),
+                          betas=list(cols[6]), damps=list(cols[7]))
+    ens.initial_state()
+    assert [int(m.lanes) for m in ens.host_meshes] == [128, 128, 8]
+    assert [int(m.packed) for m in ens.host_meshes] == [0, 0, 1]    # 16 samples: packed rows
+    ens.run_per_mesh(nsteps)
+    single = states(systems)
+    for graph in (False, True):
+        ens.groups_first = True
+        ens.initial_state()
+        ens.run(nsteps, graph=graph, block=12)
+        for a, b in zip(states(systems), single):
+            assert np.all(np.isfinite(a)) and np.max(np.abs(a)) > 0
+            assert np.array_equal(a, b), graph
 
 
 def test_graph_replays_equal_eager(require_gpu):

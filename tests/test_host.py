@@ -682,7 +682,7 @@ def test_batch_refuses_steps_past_load_tables():
     ens.systems, ens.loads = [mesh(5), mesh(7)], tables
     ens.dts, ens.methods, ens.betas, ens.damps = [1e-6] * 2, ["matfree"] * 2, [0.0] * 2, [0.0] * 2
     ens.device, ens.lib, ens._graphs = cpu, Lib(), {}
-    ens.vw_hint, ens.cap, ens.waves_per_simd, ens.n_waves = 0, False, 0, 0
+    ens.vw_hint, ens.cap, ens.waves_per_simd, ens.n_waves, ens.groups_first = 0, False, 0, 0, False
     ens._plan()
     ens.phase, ens.step_index = 0, 0
     assert ens.max_steps == 6 and ens.n_tasks == 16
