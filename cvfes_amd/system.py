@@ -433,7 +433,7 @@ class MembraneSystem:
         return e0.elapsed_time(e1) / steps
 
     def autotune_row_order(self, load, dt, beta=0.0, damp=0.0, method="stored", steps=20,
-                           candidates=(("natural", 0), ("strips", 32), ("strips", 64))):
+                           candidates=(("natural", 0), ("strips", 32), ("strips", 48), ("strips", 64))):
         """Time ``steps`` steps under each visiting order in ``candidates`` ((order, strip width))
         and keep the fastest; returns ``{"order", "width", "ms"}``.  The displacements do not depend
         on the order (every order gives the same result bit for bit), only the time does.  Leaves
