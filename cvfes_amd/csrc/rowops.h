@@ -357,6 +357,18 @@ __device__ __forceinline__ typename VecT<VW>::type ldb(__amdgpu_buffer_rsrc_t r,
 }
 
 template <int VW>
+__device__ __forceinline__ Vec<VW> to_vec(typename VecT<VW>::type t) {
+    Vec<VW> r;
+    if constexpr (VW == 2) {
+        r.v[0] = t[0];
+        r.v[1] = t[1];
+    } else {
+        r.v[0] = t;
+    }
+    return r;
+}
+
+template <int VW>
 __device__ __forceinline__ typename VecT<VW>::type splat(double x) {
     if constexpr (VW == 2) return dbl2{x, x};
     else return x;

@@ -50,6 +50,8 @@ __global__ __launch_bounds__(kBlock) void k_step_updated(StepArgs a, GeomArgs g,
     bool active;
     if (!row_and_sample<UNIFORM, VW>(a.k.n_nodes, a.k.n_smp, row, s, active)) return;
     const int64_t S = a.k.n_smp;
+    const uint32_t lo = static_cast<uint32_t>(s) * 8u;   // the lane's byte offset in every row
+    const uint32_t sb = static_cast<uint32_t>(S) * 8u;   // bytes of one entry across the samples
     double acc[VW][3] = {};
     double fl[VW][3] = {};    // unit-traction load of the row node on the moved geometry
     for (int32_t q = g.nptr[row]; q < g.nptr[row + 1]; ++q) {
@@ -63,16 +65,23 @@ __global__ __launch_bounds__(kBlock) void k_step_updated(StepArgs a, GeomArgs g,
 #pragma unroll
             for (int k = 0; k < 3; ++k) ref[b][k] = g.nodes[3 * static_cast<int64_t>(ids[b]) + k];
         // per-sample state of the element's nodes: u (and up), the geometry displacement
+        // (wave-uniform rows: buffer loads from scalar resources per node, one per-lane offset)
         double xu[VW][3][3], xg[VW][3][3];
 #pragma unroll
         for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int64_t off = (static_cast<int64_t>(ids[b]) * 3 + k) * S + s;
-                const Vec<VW> uu = load_vec<VW>(a.u + off);
-                Vec<VW> pp;
-                if constexpr (BETA) pp = load_vec<VW>(a.up + off);
-                Vec<VW> gg;
+                Vec<VW> uu, pp, gg;
+                if constexpr (UNIFORM) {
+                    const int64_t node = static_cast<int64_t>(ids[b]) * 3 * S;
+                    uu = to_vec<VW>(ldb<VW, 0>(buffer_rsrc(a.u + node, 3 * sb), lo, k * sb));
+                    if constexpr (BETA)
+                        pp = to_vec<VW>(ldb<VW, 0>(buffer_rsrc(a.up + node, 3 * sb), lo, k * sb));
+                } else {
+                    uu = load_vec<VW>(a.u + off);
+                    if constexpr (BETA) pp = load_vec<VW>(a.up + off);
+                }
                 if constexpr (!GEO_U) {
                     if (g.ug) gg = load_vec<VW>(g.ug + off);
                 }
@@ -83,7 +92,9 @@ __global__ __launch_bounds__(kBlock) void k_step_updated(StepArgs a, GeomArgs g,
                     else xg[v][b][k] = g.ug ? gg.v[v] : 0.0;
                 }
             }
-        const Vec<VW> cf = load_vec<VW>(a.k.coef + static_cast<int64_t>(e) * S + s);
+        Vec<VW> cf;
+        if constexpr (UNIFORM) cf = to_vec<VW>(ldb<VW, 0>(buffer_rsrc(a.k.coef + static_cast<int64_t>(e) * S, sb), lo, 0));
+        else cf = load_vec<VW>(a.k.coef + static_cast<int64_t>(e) * S + s);
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
             double X[3][3];
@@ -273,6 +284,8 @@ extern "C" int cvfes_step_updated(void *stream, const cvfes_system_t *sys, const
     CVFES_REQUIRE(sys->n_nodes >= 1 && sys->n_smp >= 1 && nsteps >= 0 && dt > 0.0,
                   "cvfes_step_updated: bad sizes");
     CVFES_REQUIRE(sys->n_nodes * 3 < (int64_t(1) << 31), "cvfes_step_updated: too many nodes");
+    // buffer resources of 3*S doubles (a node's state) must fit 31 bits
+    CVFES_REQUIRE(sys->n_smp < (int64_t(1) << 31) / 24, "cvfes_step_updated: too many samples");
     CVFES_REQUIRE(sys->nptr && sys->nent && sys->coef && sys->lm && geo->nodes && geo->elems,
                   "cvfes_step_updated: the system needs nptr, nent, coef, lm and the geometry");
     CVFES_REQUIRE(geo->nu > -1.0 && geo->nu < 1.0, "cvfes_step_updated: nu out of range");
