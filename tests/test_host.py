@@ -232,6 +232,17 @@ def test_abi_argument_errors_without_gpu():
     assert b"block structure" in lib.cvfes_last_error()
     assert lib.cvfes_element_matrices(None, 3, None, None, 0.3, 5 / 6, None, None, None, None) == -1
     assert lib.cvfes_element_matrices(None, 0, None, None, 0.3, 5 / 6, None, None, None, None) == 0
+    # the updated-geometry step's buffer resources hold a node's 3*S doubles in 31 bits
+    dummy = ctypes.c_void_p(16)
+    upd_sys = _lib.CvfesSystem(n_nodes=4, n_smp=(1 << 31) // 24, nptr=dummy, nent=dummy, coef=dummy,
+                               lm=dummy)
+    geo = _lib.CvfesGeometry(nodes=dummy, elems=dummy, nu=0.3, kshear=5 / 6)
+    upd = _lib.CvfesUpdate(interval=1, counter=0, has_geometry=0, ug=None)
+    state = (ctypes.c_void_p * 3)(16, 32, 48)
+    load = _lib.CvfesLoad(p=dummy)
+    assert lib.cvfes_step_updated(None, ctypes.byref(upd_sys), ctypes.byref(geo), ctypes.byref(upd),
+                                  state, ctypes.byref(load), 0, 1e-6, 0.0, 0.0) == -1
+    assert b"too many samples" in lib.cvfes_last_error()
 
 
 def test_step_kernel_resolution_without_gpu():
